@@ -1,0 +1,171 @@
+/*
+ * tc_abi.h -- C-ABI of the MI355X Taylor-Couette RK3 time-step library
+ * (libtcamd.so).
+ *
+ * The reference exposes its hot path as argument-less Fortran subroutines
+ * that mutate module globals (param.F90:4-102).  This ABI keeps those routine
+ * names and their call order; the module state becomes an opaque device
+ * context.  A Fortran host binds each entry with ISO_C_BINDING.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on error; the
+ *    message is available from tc_last_error() (thread-local).
+ *  - The context owns all device memory.  Host arrays passed in are never
+ *    retained after the call returns.
+ *  - One host thread per context.  Routine entries are asynchronous on the
+ *    context's stream; entries that return host values synchronise.
+ *  - Floating point is FP64 throughout.
+ */
+#ifndef TC_ABI_H
+#define TC_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* bou.in contents, in the order ReadInputFile.F90:17-37 reads them. */
+typedef struct tc_params {
+  int nthm, nzm, nrm, nsst, idtv, nread;
+  int ntst;
+  double tout, tmax;
+  int ireset;
+  double walltimemax;
+  double alx3, rint, rext;
+  int istr;
+  double str, lamb;
+  double virotint, virotext, sizepert, omegapert;
+  double ren, romeg, resid, limitcfl;
+  double tsta;
+  int starea;
+  double dtmin, dtmax, limitvel, cfllim;
+  int nslabr, dpthcut, dpstave;
+  double tframe2d;
+  int dpfullf;
+  double tframe3d;
+  int minstrt, minstrz, multmin;
+} tc_params;
+
+typedef struct tc_ctx tc_ctx;
+
+/* field ids: the local_arrays module (param.F90:97-100) */
+enum {
+  TC_QT = 0, TC_QR = 1, TC_QZ = 2, TC_PR = 3, TC_DPH = 4, TC_RHS = 5,
+  TC_DQ = 6, TC_QCAP = 7, TC_RU1 = 8, TC_RU2 = 9, TC_RU3 = 10, TC_NFIELDS = 11
+};
+
+/* host array layouts for upload/download */
+enum {
+  TC_LAYOUT_FORTRAN = 0, /* (1:nr,1:nzm,1:nthm) column-major, r fastest */
+  TC_LAYOUT_DEVICE = 1   /* [r][z][theta], theta fastest (device order)  */
+};
+
+/* grid-vector ids for tc_grid_vector (out[q] = v(q+1), Fortran 1-based) */
+enum {
+  TC_V_RC = 0, TC_V_RM, TC_V_G2RC, TC_V_G2RM, TC_V_USRC, TC_V_USRM,
+  TC_V_UDX2RM, TC_V_UDX2M, TC_V_UDX2C, TC_V_RUDX2C, TC_V_UDX1VIS1,
+  TC_V_UDX1VIS2, TC_V_UDX1QM, TC_V_AP1J, TC_V_AM1J, TC_V_AC1J, TC_V_AP2J,
+  TC_V_AM2J, TC_V_AC2J, TC_V_AP2JE, TC_V_AM2JE, TC_V_AC2JE, TC_V_AP3J,
+  TC_V_AM3J, TC_V_AC3J, TC_V_APPH, TC_V_AMPH, TC_V_ACPH, TC_V_AK1, TC_V_AK3,
+  TC_V_FS, TC_V_PVOL, TC_NVEC
+};
+
+/* ---------------- lifecycle and state ---------------- */
+
+/* InitTimeMarchScheme + InitVariables + CreateGrid(+SlipBCs) +
+ * InitPressureSolver: main.F90:70-94 (InitTimeMarchScheme.F90:16-48,
+ * InitVariables.F90:26-57, CreateGrid.F90:28-323, InitPressureSolver.F90:22-65).
+ * Allocates the device state; all fields start at zero
+ * (AuxiliaryRoutines.F90:123). */
+int tc_create(const tc_params *p, tc_ctx **out);
+int tc_destroy(tc_ctx *ctx);
+const char *tc_last_error(void);
+
+/* Host-only CreateGrid/InitPressureSolver vectors (no device needed).
+ * Returns the number of values written (<= n) or a negative error. */
+int tc_grid_vector(const tc_params *p, int id, double *out, int n);
+
+/* Copy a field in or out; replaces the reference's direct access to the
+ * local_arrays module and the restart read (ReadFlowField.F90:135-141).
+ * Halos do not exist on the device: z and theta are wrapped periodically. */
+int tc_upload_field(tc_ctx *ctx, int field, const double *host, int layout);
+int tc_download_field(tc_ctx *ctx, int field, double *host, int layout);
+
+/* icylsl/ocylsl masks (nzm,nthm) column-major (param.F90:59). */
+int tc_set_masks(tc_ctx *ctx, const int *icylsl, const int *ocylsl);
+int tc_get_masks(tc_ctx *ctx, int *icylsl, int *ocylsl);
+
+/* The scalars time and dt of module param (param.F90:15,29). */
+int tc_set_time(tc_ctx *ctx, double time, double dt);
+/* out = time dt beta al ga ro vinner_t dvinner_t */
+int tc_get_scalars(tc_ctx *ctx, double *out8);
+
+/* TimeMarcher.F90:20-23: beta=dt/(2 Re), InnerCylVelocity, SlipBCs. */
+int tc_begin_step(tc_ctx *ctx);
+/* TimeMarcher.F90:27-29: al, ga, ro of substep ns (1-based). */
+int tc_set_substep(tc_ctx *ctx, int ns);
+
+/* ---------------- the hot-path routines ---------------- */
+
+int tc_explicit_terms_qt(tc_ctx *ctx);       /* ExplicitTermsQT.F90:78-133      */
+int tc_explicit_terms_qr(tc_ctx *ctx);       /* ExplicitTermsQR.F90:214-353     */
+int tc_explicit_terms_qz(tc_ctx *ctx);       /* ExplicitTermsQZ.F90:366-465     */
+int tc_implicit_and_update_qt(tc_ctx *ctx);  /* ImplicitAndUpdateQT.F90:14-124 +
+                                                SolveImpEqnUpdate_QT.F90:136-208 */
+int tc_implicit_and_update_qr(tc_ctx *ctx);  /* ImplicitAndUpdateQR.F90:223-283 +
+                                                SolveImpEqnUpdate_QR.F90:295-355 */
+int tc_implicit_and_update_qz(tc_ctx *ctx);  /* ImplicitAndUpdateQZ.F90:14-123 +
+                                                SolveImpEqnUpdate_QZ.F90:134-197 */
+int tc_calc_local_divergence(tc_ctx *ctx);   /* CalcLocalDivergence.F90:11-44   */
+int tc_solve_pressure_correction(tc_ctx *ctx);/* SolvePressureCorrection.F90:56-217 */
+int tc_correct_velocity(tc_ctx *ctx);        /* CorrectVelocity.F90:229-267     */
+int tc_correct_pressure(tc_ctx *ctx);        /* CorrectPressure.F90:279-320     */
+
+/* TimeMarcher.F90:12-57: one RK3 (or AB2) step at the current dt,
+ * including tc_begin_step.  Uses fused kernels unless tc_set_fused(ctx,0). */
+int tc_time_marcher(tc_ctx *ctx);
+int tc_set_fused(tc_ctx *ctx, int fused);
+
+/* ---------------- reductions (synchronise) ---------------- */
+
+int tc_calc_max_cfl(tc_ctx *ctx, double *cflm);        /* CalcMaxCFL.F90:12-49     */
+int tc_check_divergence(tc_ctx *ctx, double *qmax);    /* CheckDivergence.F90:60-95 */
+int tc_calc_torque(tc_ctx *ctx, double *inner, double *outer); /* CalcTorque.F90:107-157 */
+int tc_calc_velocity_stats(tc_ctx *ctx, double *vmax3, double *rms3); /* CalcVelocityStats.F90:11-68 */
+
+/* ---------------- driver helpers ---------------- */
+
+/* One iteration of the main.F90:152-259 loop body: CFL, dt rule (fixed-dt
+ * mode uses dt = DTMAX), TimeMarcher, time advance, logging-step
+ * diagnostics and the error codes 165/166/169/266/333/334/555.
+ * out9 = cfl dt time dmax torque_in torque_out vmax1 vmax2 vmax3.
+ * Returns the error code (>= 0) or a negative ABI error. */
+int tc_step(tc_ctx *ctx, int ntime, double walltime, double *out9);
+
+/* Divergence-free projection of the current velocity with al = dt = 1
+ * (CalcLocalDivergence + SolvePressureCorrection + CorrectVelocity), used
+ * to build starting fields; pr is left untouched. */
+int tc_project(tc_ctx *ctx);
+
+/* Seeded starting field in Fortran layout (nr,nzm,nthm): laminar Couette
+ * qt = A rm + B/rm (CreateInitialConditions.F90:180-189) plus i.i.d.
+ * U(-amp,amp) on qt (k=1..nrm), qr (k=2..nrm) and qz (k=1..nrm) from a
+ * splitmix64 stream; the k=nr plane and the qr walls are zero. */
+int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed,
+                              double amp, double *qt, double *qr, double *qz);
+
+int tc_synchronize(tc_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context stream.  When enabled,
+ * each launch of a timed kernel is bracketed by two events; read returns the
+ * summed milliseconds and launch count since the last reset. */
+int tc_kernel_timing(tc_ctx *ctx, int enable);
+int tc_kernel_timing_read(tc_ctx *ctx, const char *kernel, double *total_ms,
+                          long long *count, int reset);
+
+/* Device bytes held by the context (fields + scratch). */
+int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TC_ABI_H */

@@ -1,0 +1,302 @@
+"""Python view of libtcamd.so, the MI355X Taylor-Couette RK3 time-step library.
+
+The C-ABI (include/tc_abi.h) is the product boundary; this module is a thin
+ctypes binding used by the tests and the benchmark.  `Solver` exposes the
+reference's routine names (ExplicitTermsQT, ImplicitAndUpdateQR,
+SolvePressureCorrection, ...) on a device-resident context.
+
+Fields are exchanged as numpy arrays of shape (nr, nzm, nthm) in Fortran
+order, the reference's (1:nr,1:nzm,1:nthm) without halos.  Importing works
+without a GPU; creating a Solver needs one and raises if none is visible or
+if the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import configs  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libtcamd.so")
+
+FIELDS = {"qt": 0, "qr": 1, "qz": 2, "pr": 3, "dph": 4, "rhs": 5, "dq": 6,
+          "qcap": 7, "ru1": 8, "ru2": 9, "ru3": 10}
+VECS = ["rc", "rm", "g2rc", "g2rm", "usrc", "usrm", "udx2rm", "udx2m", "udx2c",
+        "rudx2c", "udx1vis1", "udx1vis2", "udx1qm", "ap1j", "am1j", "ac1j",
+        "ap2j", "am2j", "ac2j", "ap2je", "am2je", "ac2je", "ap3j", "am3j",
+        "ac3j", "apph", "amph", "acph", "ak1", "ak3", "fs", "pvol"]
+LAYOUT_FORTRAN, LAYOUT_DEVICE = 0, 1
+
+PARAM_FIELDS = [
+    ("nthm", ctypes.c_int), ("nzm", ctypes.c_int), ("nrm", ctypes.c_int),
+    ("nsst", ctypes.c_int), ("idtv", ctypes.c_int), ("nread", ctypes.c_int),
+    ("ntst", ctypes.c_int), ("tout", ctypes.c_double), ("tmax", ctypes.c_double),
+    ("ireset", ctypes.c_int), ("walltimemax", ctypes.c_double),
+    ("alx3", ctypes.c_double), ("rint", ctypes.c_double), ("rext", ctypes.c_double),
+    ("istr", ctypes.c_int), ("str", ctypes.c_double), ("lamb", ctypes.c_double),
+    ("virotint", ctypes.c_double), ("virotext", ctypes.c_double),
+    ("sizepert", ctypes.c_double), ("omegapert", ctypes.c_double),
+    ("ren", ctypes.c_double), ("romeg", ctypes.c_double), ("resid", ctypes.c_double),
+    ("limitcfl", ctypes.c_double), ("tsta", ctypes.c_double), ("starea", ctypes.c_int),
+    ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double), ("limitvel", ctypes.c_double),
+    ("cfllim", ctypes.c_double), ("nslabr", ctypes.c_int), ("dpthcut", ctypes.c_int),
+    ("dpstave", ctypes.c_int), ("tframe2d", ctypes.c_double), ("dpfullf", ctypes.c_int),
+    ("tframe3d", ctypes.c_double), ("minstrt", ctypes.c_int), ("minstrz", ctypes.c_int),
+    ("multmin", ctypes.c_int),
+]
+
+# every entry point declared in include/tc_abi.h
+ABI_SYMBOLS = [
+    "tc_create", "tc_destroy", "tc_last_error", "tc_grid_vector", "tc_upload_field",
+    "tc_download_field", "tc_set_masks", "tc_get_masks", "tc_set_time", "tc_get_scalars",
+    "tc_begin_step", "tc_set_substep", "tc_explicit_terms_qt", "tc_explicit_terms_qr",
+    "tc_explicit_terms_qz", "tc_implicit_and_update_qt", "tc_implicit_and_update_qr",
+    "tc_implicit_and_update_qz", "tc_calc_local_divergence", "tc_solve_pressure_correction",
+    "tc_correct_velocity", "tc_correct_pressure", "tc_time_marcher", "tc_set_fused",
+    "tc_calc_max_cfl", "tc_check_divergence", "tc_calc_torque", "tc_calc_velocity_stats",
+    "tc_step", "tc_project", "tc_init_perturbed_couette", "tc_synchronize",
+    "tc_kernel_timing", "tc_kernel_timing_read", "tc_device_bytes",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+def make_params(d):
+    p = Params()
+    for name, _ in PARAM_FIELDS:
+        setattr(p, name, d.get(name, 0))
+    return p
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_V = ctypes.c_void_p
+_LIB = None
+
+
+class TCError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libtcamd.so; raises if it has not been built (no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise TCError(f"{LIBPATH} is missing: build it with __graft_entry__.build() "
+                      "or python taylor-couette-afid_amd/build.py")
+    L = ctypes.CDLL(LIBPATH)
+    L.tc_last_error.restype = ctypes.c_char_p
+    L.tc_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(_V)]
+    L.tc_destroy.argtypes = [_V]
+    L.tc_grid_vector.argtypes = [ctypes.POINTER(Params), ctypes.c_int, _D, ctypes.c_int]
+    L.tc_upload_field.argtypes = [_V, ctypes.c_int, _D, ctypes.c_int]
+    L.tc_download_field.argtypes = [_V, ctypes.c_int, _D, ctypes.c_int]
+    L.tc_set_masks.argtypes = [_V, _I, _I]
+    L.tc_get_masks.argtypes = [_V, _I, _I]
+    L.tc_set_time.argtypes = [_V, ctypes.c_double, ctypes.c_double]
+    L.tc_get_scalars.argtypes = [_V, _D]
+    L.tc_set_substep.argtypes = [_V, ctypes.c_int]
+    L.tc_set_fused.argtypes = [_V, ctypes.c_int]
+    for n in ["tc_begin_step", "tc_explicit_terms_qt", "tc_explicit_terms_qr",
+              "tc_explicit_terms_qz", "tc_implicit_and_update_qt", "tc_implicit_and_update_qr",
+              "tc_implicit_and_update_qz", "tc_calc_local_divergence",
+              "tc_solve_pressure_correction", "tc_correct_velocity", "tc_correct_pressure",
+              "tc_time_marcher", "tc_project", "tc_synchronize"]:
+        getattr(L, n).argtypes = [_V]
+    L.tc_calc_max_cfl.argtypes = [_V, _D]
+    L.tc_check_divergence.argtypes = [_V, _D]
+    L.tc_calc_torque.argtypes = [_V, _D, _D]
+    L.tc_calc_velocity_stats.argtypes = [_V, _D, _D]
+    L.tc_step.argtypes = [_V, ctypes.c_int, ctypes.c_double, _D]
+    L.tc_init_perturbed_couette.argtypes = [ctypes.POINTER(Params), ctypes.c_ulonglong,
+                                            ctypes.c_double, _D, _D, _D]
+    L.tc_kernel_timing.argtypes = [_V, ctypes.c_int]
+    L.tc_kernel_timing_read.argtypes = [_V, ctypes.c_char_p, _D,
+                                        ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    L.tc_device_bytes.argtypes = [_V, ctypes.POINTER(ctypes.c_ulonglong)]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise TCError(lib().tc_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def grid_vector(params, name):
+    """CreateGrid / InitPressureSolver vector computed by the library (host only)."""
+    p = make_params(params)
+    n = max(params["nrm"] + 1, params["nthm"], params["nzm"], 16)
+    out = np.zeros(n)
+    m = _check(lib().tc_grid_vector(ctypes.byref(p), VECS.index(name), _dp(out), n))
+    return out[:m].copy()
+
+
+def init_perturbed_couette(params, seed, amp):
+    """Seeded starting velocity (qt, qr, qz), each (nr, nzm, nthm) Fortran order."""
+    p = make_params(params)
+    shape = (params["nrm"] + 1, params["nzm"], params["nthm"])
+    qt, qr, qz = (np.zeros(shape, order="F") for _ in range(3))
+    _check(lib().tc_init_perturbed_couette(ctypes.byref(p), seed, amp, _dp(qt), _dp(qr), _dp(qz)))
+    return qt, qr, qz
+
+
+class Solver:
+    """One device-resident Taylor-Couette instance with the AFiD routine names."""
+
+    def __init__(self, params):
+        self.params = dict(params)
+        self._p = make_params(self.params)
+        h = _V()
+        _check(lib().tc_create(ctypes.byref(self._p), ctypes.byref(h)))
+        self.h = h
+        self.nthm, self.nzm, self.nrm = params["nthm"], params["nzm"], params["nrm"]
+        self.nr = self.nrm + 1
+        self.shape = (self.nr, self.nzm, self.nthm)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- state ---
+    def set(self, name, arr):
+        a = np.asfortranarray(np.asarray(arr, dtype=np.float64).reshape(self.shape, order="F"))
+        _check(lib().tc_upload_field(self.h, FIELDS[name], _dp(a), LAYOUT_FORTRAN))
+
+    def get(self, name):
+        out = np.zeros(self.shape, order="F")
+        _check(lib().tc_download_field(self.h, FIELDS[name], _dp(out), LAYOUT_FORTRAN))
+        return out
+
+    def set_masks(self, icyl, ocyl):
+        i = np.asfortranarray(icyl, dtype=np.int32)
+        o = np.asfortranarray(ocyl, dtype=np.int32)
+        _check(lib().tc_set_masks(self.h, i.ctypes.data_as(_I), o.ctypes.data_as(_I)))
+
+    def masks(self):
+        i = np.zeros((self.nzm, self.nthm), dtype=np.int32, order="F")
+        o = np.zeros((self.nzm, self.nthm), dtype=np.int32, order="F")
+        _check(lib().tc_get_masks(self.h, i.ctypes.data_as(_I), o.ctypes.data_as(_I)))
+        return i, o
+
+    def set_time(self, time, dt):
+        _check(lib().tc_set_time(self.h, time, dt))
+
+    def scalars(self):
+        o = np.zeros(8)
+        _check(lib().tc_get_scalars(self.h, _dp(o)))
+        return dict(zip(["time", "dt", "beta", "al", "ga", "ro", "vinner_t", "dvinner_t"], o))
+
+    def set_fused(self, on):
+        _check(lib().tc_set_fused(self.h, int(bool(on))))
+
+    def synchronize(self):
+        _check(lib().tc_synchronize(self.h))
+
+    # --- reference routines (TimeMarcher.F90:20-54 call order) ---
+    def begin_step(self):
+        _check(lib().tc_begin_step(self.h))
+
+    def set_substep(self, ns):
+        _check(lib().tc_set_substep(self.h, ns))
+
+    def ExplicitTermsQT(self):
+        _check(lib().tc_explicit_terms_qt(self.h))
+
+    def ExplicitTermsQR(self):
+        _check(lib().tc_explicit_terms_qr(self.h))
+
+    def ExplicitTermsQZ(self):
+        _check(lib().tc_explicit_terms_qz(self.h))
+
+    def ImplicitAndUpdateQT(self):
+        _check(lib().tc_implicit_and_update_qt(self.h))
+
+    def ImplicitAndUpdateQR(self):
+        _check(lib().tc_implicit_and_update_qr(self.h))
+
+    def ImplicitAndUpdateQZ(self):
+        _check(lib().tc_implicit_and_update_qz(self.h))
+
+    def CalcLocalDivergence(self):
+        _check(lib().tc_calc_local_divergence(self.h))
+
+    def SolvePressureCorrection(self):
+        _check(lib().tc_solve_pressure_correction(self.h))
+
+    def CorrectVelocity(self):
+        _check(lib().tc_correct_velocity(self.h))
+
+    def CorrectPressure(self):
+        _check(lib().tc_correct_pressure(self.h))
+
+    def TimeMarcher(self):
+        _check(lib().tc_time_marcher(self.h))
+
+    def CalcMaxCFL(self):
+        v = ctypes.c_double()
+        _check(lib().tc_calc_max_cfl(self.h, ctypes.byref(v)))
+        return v.value
+
+    def CheckDivergence(self):
+        v = ctypes.c_double()
+        _check(lib().tc_check_divergence(self.h, ctypes.byref(v)))
+        return v.value
+
+    def CalcTorque(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().tc_calc_torque(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def CalcVelocityStats(self):
+        v, r = np.zeros(3), np.zeros(3)
+        _check(lib().tc_calc_velocity_stats(self.h, _dp(v), _dp(r)))
+        return v, r
+
+    def project(self):
+        _check(lib().tc_project(self.h))
+
+    def step(self, ntime, walltime=0.0):
+        """One main.F90 loop iteration; returns (errorcode, diagnostics)."""
+        o = np.zeros(9)
+        code = _check(lib().tc_step(self.h, ntime, walltime, _dp(o)))
+        return code, dict(zip(["cfl", "dt", "time", "dmax", "torque_in", "torque_out",
+                               "vmax1", "vmax2", "vmax3"], o))
+
+    # --- instrumentation ---
+    def kernel_timing(self, on):
+        _check(lib().tc_kernel_timing(self.h, int(bool(on))))
+
+    def kernel_time(self, name, reset=False):
+        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().tc_kernel_timing_read(self.h, name.encode(), ctypes.byref(ms),
+                                           ctypes.byref(cnt), int(bool(reset))))
+        return ms.value, cnt.value
+
+    def device_bytes(self):
+        v = ctypes.c_ulonglong()
+        _check(lib().tc_device_bytes(self.h, ctypes.byref(v)))
+        return v.value

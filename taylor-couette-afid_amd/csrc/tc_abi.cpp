@@ -1,0 +1,774 @@
+// C-ABI of libtcamd.so: device context, state transfer and the AFiD routine
+// entries.  See include/tc_abi.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "tc_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+  g_err = msg;
+  return -1;
+}
+
+#define HIPCHK(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      g_err = std::string(#call) + ": " + hipGetErrorString(e_);                         \
+      return -2;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+#define CTXCHK(c)                                   \
+  do {                                              \
+    if (!(c)) return fail("null tc_ctx");           \
+    if (hipSetDevice((c)->device) != hipSuccess)    \
+      return fail("hipSetDevice failed");           \
+  } while (0)
+
+int launch_status() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_err = std::string("kernel launch: ") + hipGetErrorString(e);
+    return -2;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Helmholtz LU tables (SolveImpEqnUpdate_QT.F90:161-187,
+// SolveImpEqnUpdate_QR.F90:310-327, SolveImpEqnUpdate_QZ.F90:158-181)
+// ---------------------------------------------------------------------------
+void write_table(double *T, int nt, int n, std::vector<double> &dl, std::vector<double> &d,
+                 std::vector<double> &du, const std::vector<double> &sc) {
+  const int stride = nt / 6;
+  std::vector<double> du2((size_t)std::max(n, 1), 0.0);
+  std::vector<int> ipiv((size_t)n);
+  tc::dgttrf(n, dl.data(), d.data(), du.data(), du2.data(), ipiv.data());
+  std::fill(T, T + nt, 0.0);
+  for (int q = 0; q < n - 1; q++) {
+    T[q] = dl[q];
+    T[stride + q] = du[q];
+    T[5 * stride + q] = (ipiv[q] != q) ? 1.0 : 0.0;
+  }
+  for (int q = 0; q < n - 2; q++) T[2 * stride + q] = du2[q];
+  for (int q = 0; q < n; q++) {
+    T[3 * stride + q] = 1.0 / d[q];
+    T[4 * stride + q] = sc[q];
+  }
+}
+
+// Row-normalised matrix of the q_theta / q_z solves (mask variant ic, oc).
+void build_wall_table(const tc::Grid &g, int comp, int ic, int oc, double betadx, double *T, int nt) {
+  const int n = g.nrm;
+  const bool is_qt = (comp == tc::LU_QT);
+  const std::vector<double> &ap = is_qt ? g.ap1j : g.ap3j;
+  const std::vector<double> &am = is_qt ? g.am1j : g.am3j;
+  const std::vector<double> &ac = is_qt ? g.ac1j : g.ac3j;
+  const double ac_fs1 = is_qt ? g.ac1j_fs[1] : g.ac3j_fs[1];
+  const double ac_fs2 = is_qt ? g.ac1j_fs[2] : g.ac3j_fs[2];
+  std::vector<double> amkl((size_t)n + 2), ackl((size_t)n + 2), apkl((size_t)n + 2);
+  const double fic = (double)ic, foc = (double)oc;
+  double actot = (ac[1] * fic + ac_fs1 * (1 - fic));
+  double ackl_b = 1.0 / (1.0 - actot * betadx);
+  ackl[1] = 1.0;
+  apkl[1] = -ap[1] * betadx * ackl_b;
+  for (int kc = 2; kc <= n - 1; kc++) {
+    ackl_b = 1.0 / (1.0 - ac[kc] * betadx);
+    amkl[kc] = -am[kc] * betadx * ackl_b;
+    ackl[kc] = 1.0;
+    apkl[kc] = -ap[kc] * betadx * ackl_b;
+  }
+  actot = (ac[n] * foc + ac_fs2 * (1 - foc));
+  ackl_b = 1.0 / (1.0 - actot * betadx);
+  amkl[n] = -am[n] * betadx * ackl_b;
+  ackl[n] = 1.0;
+  apkl[n] = 0.0;
+  std::vector<double> dl((size_t)n), d((size_t)n), du((size_t)n), sc((size_t)n);
+  for (int q = 0; q < n - 1; q++) {
+    dl[q] = amkl[q + 2];
+    du[q] = apkl[q + 1];
+  }
+  for (int q = 0; q < n; q++) d[q] = ackl[q + 1];
+  // right-hand side row scaling uses the unblended diagonal
+  for (int kc = 1; kc <= n; kc++) sc[kc - 1] = 1.0 / (1. - ac[kc] * betadx);
+  write_table(T, nt, n, dl, d, du, sc);
+}
+
+void build_qr_table(const tc::Grid &g, double betadx, double *T, int nt) {
+  const int n = g.nr, nrm = g.nrm;
+  std::vector<double> amkl((size_t)n + 2), ackl((size_t)n + 2), apkl((size_t)n + 2);
+  amkl[1] = 0.;
+  apkl[1] = 0.;
+  ackl[1] = 1.;
+  for (int kc = 2; kc <= nrm; kc++) {
+    const double ackl_b = 1.0 / (1.0 - g.ac2j[kc] * betadx);
+    amkl[kc] = -g.am2j[kc] * betadx * ackl_b;
+    ackl[kc] = 1.0;
+    apkl[kc] = -g.ap2j[kc] * betadx * ackl_b;
+  }
+  amkl[n] = 0.;
+  apkl[n] = 0.;
+  ackl[n] = 1.;
+  std::vector<double> dl((size_t)n), d((size_t)n), du((size_t)n), sc((size_t)n, 0.0);
+  for (int q = 0; q < n - 1; q++) {
+    dl[q] = amkl[q + 2];
+    du[q] = apkl[q + 1];
+  }
+  for (int q = 0; q < n; q++) d[q] = ackl[q + 1];
+  for (int kc = 2; kc <= nrm; kc++) sc[kc - 1] = 1.0 / (1.0 - g.ac2j[kc] * betadx);
+  write_table(T, nt, n, dl, d, du, sc);
+}
+
+int refresh_tables(tc_ctx *c) {
+  const int nsst = c->p.nsst;
+  bool any = !c->lu_valid;
+  for (int s = 0; s < nsst; s++)
+    if (c->lu_key[s] != c->beta * c->g.alm[s + 1]) any = true;
+  if (!any) return 0;
+  // the previous upload must have left the staging buffer
+  HIPCHK(hipEventSynchronize(c->lu_copied));
+  const size_t per_slot = (size_t)3 * 4 * c->lu_nt;
+  for (int s = 0; s < nsst; s++) {
+    const double betadx = c->beta * c->g.alm[s + 1];
+    double *base = c->lu_host + (size_t)s * per_slot;
+    for (int v = 0; v < 4; v++) {
+      build_wall_table(c->g, tc::LU_QT, v & 1, v >> 1, betadx,
+                       base + ((size_t)tc::LU_QT * 4 + v) * c->lu_nt, c->lu_nt);
+      build_wall_table(c->g, tc::LU_QZ, v & 1, v >> 1, betadx,
+                       base + ((size_t)tc::LU_QZ * 4 + v) * c->lu_nt, c->lu_nt);
+    }
+    build_qr_table(c->g, betadx, base + (size_t)tc::LU_QR * 4 * c->lu_nt, c->lu_nt);
+    c->lu_key[s] = betadx;
+  }
+  HIPCHK(hipMemcpyAsync(c->lu, c->lu_host, sizeof(double) * per_slot * nsst, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipEventRecord(c->lu_copied, c->stream));
+  c->lu_valid = true;
+  return 0;
+}
+
+int upload_masks(tc_ctx *c) {
+  std::vector<unsigned char> m((size_t)2 * c->P);
+  bool ones = true;
+  for (int i = 0; i < c->nthm; i++)
+    for (int j = 0; j < c->nzm; j++) {
+      const size_t f = (size_t)j + (size_t)c->nzm * i;
+      const size_t d = (size_t)j * c->nthm + i;
+      m[d] = (unsigned char)(c->icyl[f] != 0);
+      m[c->P + d] = (unsigned char)(c->ocyl[f] != 0);
+      ones = ones && c->icyl[f] == 1 && c->ocyl[f] == 1;
+    }
+  c->masks_uniform_one = ones;
+  HIPCHK(hipMemcpyAsync(c->mask, m.data(), m.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// SlipBCs at the current time; re-uploads only when the pattern can change
+int apply_slip_bcs(tc_ctx *c) {
+  if (c->p.minstrt * c->p.multmin + c->p.minstrz * c->p.multmin == 0 && c->masks_uniform_one)
+    return 0;
+  tc::slip_bcs(c->p, c->g, c->time, c->icyl, c->ocyl);
+  return upload_masks(c);
+}
+
+int dev_alloc(tc_ctx *c, void **p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes));
+  c->device_bytes += bytes;
+  return 0;
+}
+
+void destroy_ctx(tc_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (int q = 0; q < TC_NFIELDS; q++)
+    if (c->f[q]) hipFree(c->f[q]);
+  if (c->spec) hipFree(c->spec);
+  if (c->cpw) hipFree(c->cpw);
+  if (c->gvbuf) hipFree(c->gvbuf);
+  if (c->mask) hipFree(c->mask);
+  if (c->lu) hipFree(c->lu);
+  if (c->lu_host) hipHostFree(c->lu_host);
+  if (c->lu_copied) hipEventDestroy(c->lu_copied);
+  if (c->tw_t) hipFree(c->tw_t);
+  if (c->tw_z) hipFree(c->tw_z);
+  if (c->red) hipFree(c->red);
+  if (c->red_out) hipFree(c->red_out);
+  if (c->red_host) hipHostFree(c->red_host);
+  for (auto &t : c->timers) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  for (auto e : c->event_pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int read_reduction(tc_ctx *c, int nv, double *out) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nv; q++) out[q] = c->red_host[q];
+  return 0;
+}
+
+int run_substep(tc_ctx *c) {
+  if (c->fused) {
+    tc::launch_explicit_all(c);
+  } else {
+    tc::launch_explicit_qt(c);
+    tc::launch_explicit_qr(c);
+    tc::launch_explicit_qz(c);
+  }
+  tc::launch_implicit_qt(c);
+  tc::launch_implicit_qr(c);
+  tc::launch_implicit_qz(c);
+  tc::launch_divergence(c, 1.0 / (c->al * c->dt));
+  tc::launch_poisson(c);
+  if (c->fused) {
+    tc::launch_correct_both(c);
+  } else {
+    tc::launch_correct_velocity(c, c->al);
+    tc::launch_correct_pressure(c);
+  }
+  return launch_status();
+}
+
+int begin_step(tc_ctx *c) {
+  // TimeMarcher.F90:20-23
+  c->beta = c->dt / c->p.ren * 0.5;
+  const double pi = c->g.pi;
+  c->vinner_t = c->p.sizepert * std::sin(2. * pi * c->p.omegapert * c->time);
+  c->dvinner_t = 2. * pi * c->p.omegapert * c->p.sizepert * std::cos(2. * pi * c->p.omegapert * c->time) * c->dt;
+  int rc = apply_slip_bcs(c);
+  if (rc) return rc;
+  return refresh_tables(c);
+}
+
+void set_substep(tc_ctx *c, int ns) {
+  c->ns = ns;
+  c->al = c->g.alm[ns];
+  c->ga = c->g.gam[ns];
+  c->ro = c->g.rom[ns];
+}
+
+}  // namespace
+
+namespace tc {
+
+void timer_begin(tc_ctx *c, const char *name) {
+  if (!c->timing) return;
+  Timer t;
+  t.name = name;
+  for (int q = 0; q < 2; q++) {
+    hipEvent_t e;
+    if (!c->event_pool.empty()) {
+      e = c->event_pool.back();
+      c->event_pool.pop_back();
+    } else {
+      hipEventCreate(&e);
+    }
+    (q == 0 ? t.a : t.b) = e;
+  }
+  hipEventRecord(t.a, c->stream);
+  c->timers.push_back(t);
+}
+
+void timer_end(tc_ctx *c) {
+  if (!c->timing || c->timers.empty()) return;
+  hipEventRecord(c->timers.back().b, c->stream);
+}
+
+}  // namespace tc
+
+extern "C" {
+
+const char *tc_last_error(void) { return g_err.c_str(); }
+
+int tc_grid_vector(const tc_params *p, int id, double *out, int n) {
+  if (!p || !out) return fail("null argument");
+  tc::Grid g;
+  std::string e = tc::build_grid(*p, g);
+  if (!e.empty()) return fail(e);
+  int m = tc::grid_vector(g, id, out, n);
+  if (m < 0) return fail("unknown grid vector id");
+  return m;
+}
+
+int tc_create(const tc_params *p, tc_ctx **out) {
+  if (!p || !out) return fail("null argument");
+  *out = nullptr;
+  tc_ctx *c = new (std::nothrow) tc_ctx();
+  if (!c) return fail("out of host memory");
+  c->p = *p;
+  std::string e = tc::build_grid(*p, c->g);
+  if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, 64);
+  if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, 16);
+  if (!e.empty()) {
+    delete c;
+    return fail(e);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    delete c;
+    return fail("no HIP device available");
+  }
+  hipGetDevice(&c->device);
+  c->nthm = p->nthm;
+  c->nzm = p->nzm;
+  c->nrm = p->nrm;
+  c->nr = p->nrm + 1;
+  c->nth2 = p->nthm / 2 + 1;
+  c->P = (long long)c->nzm * c->nthm;
+  c->N = (long long)c->nr * c->P;
+  int rc = 0;
+#define TRY(x)              \
+  do {                      \
+    rc = (x);               \
+    if (rc) {               \
+      destroy_ctx(c);       \
+      return rc;            \
+    }                       \
+  } while (0)
+#define HTRY(x)                                                          \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      g_err = std::string(#x) + ": " + hipGetErrorString(e_);            \
+      destroy_ctx(c);                                                    \
+      return -2;                                                         \
+    }                                                                    \
+  } while (0)
+  HTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int q = 0; q < TC_NFIELDS; q++) {
+    TRY(dev_alloc(c, (void **)&c->f[q], sizeof(double) * (size_t)c->N));
+    HTRY(hipMemsetAsync(c->f[q], 0, sizeof(double) * (size_t)c->N, c->stream));
+  }
+  const size_t M = (size_t)c->nzm * c->nth2;
+  TRY(dev_alloc(c, (void **)&c->spec, sizeof(double2) * M * c->nrm));
+  TRY(dev_alloc(c, (void **)&c->cpw, sizeof(double) * M * c->nrm));
+  // grid vectors
+  {
+    const tc::Grid &g = c->g;
+    const std::vector<double> *vs[] = {&g.rc,   &g.rm,    &g.usrc,  &g.usrm,  &g.udx2rm,
+                                       &g.udx2m, &g.udx2c, &g.rudx2c, &g.udx1vis1, &g.udx1vis2,
+                                       &g.udx1qm, &g.pvol, &g.ap1j,  &g.am1j,  &g.ac1j,
+                                       &g.ap2je, &g.am2je, &g.ac2je, &g.ap3j,  &g.am3j,
+                                       &g.ac3j,  &g.apph,  &g.amph,  &g.acph,  &g.ak1, &g.ak3};
+    const int nv = (int)(sizeof vs / sizeof vs[0]);
+    size_t stride = (size_t)std::max(c->nr, std::max(c->nthm, c->nzm)) + 2;
+    std::vector<double> h(stride * nv, 0.0);
+    for (int q = 0; q < nv; q++) std::copy(vs[q]->begin(), vs[q]->end(), h.begin() + q * stride);
+    TRY(dev_alloc(c, (void **)&c->gvbuf, sizeof(double) * h.size()));
+    HTRY(hipMemcpy(c->gvbuf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    const double **dst[] = {&c->gd.rc,   &c->gd.rm,    &c->gd.usrc,  &c->gd.usrm,  &c->gd.udx2rm,
+                            &c->gd.udx2m, &c->gd.udx2c, &c->gd.rudx2c, &c->gd.udx1vis1,
+                            &c->gd.udx1vis2, &c->gd.udx1qm, &c->gd.pvol, &c->gd.ap1j,
+                            &c->gd.am1j, &c->gd.ac1j, &c->gd.ap2je, &c->gd.am2je, &c->gd.ac2je,
+                            &c->gd.ap3j, &c->gd.am3j, &c->gd.ac3j, &c->gd.apph, &c->gd.amph,
+                            &c->gd.acph, &c->gd.ak1, &c->gd.ak3};
+    for (int q = 0; q < nv; q++) *dst[q] = c->gvbuf + q * stride;
+    c->gd.dxt = g.dxt;
+    c->gd.dxz = g.dxz;
+    c->gd.dxtq = g.dxtq;
+    c->gd.dxzq = g.dxzq;
+    c->gd.ren = p->ren;
+    c->gd.romeg = p->romeg;
+    c->gd.ap1j_fs2 = g.ap1j_fs[2];
+    c->gd.ac1j_fs1 = g.ac1j_fs[1];
+    c->gd.ac1j_fs2 = g.ac1j_fs[2];
+    c->gd.am1j_fs1 = g.am1j_fs[1];
+    c->gd.ap3j_fs2 = g.ap3j_fs[2];
+    c->gd.ac3j_fs1 = g.ac3j_fs[1];
+    c->gd.ac3j_fs2 = g.ac3j_fs[2];
+    c->gd.am3j_fs1 = g.am3j_fs[1];
+    c->gd.nthm = c->nthm;
+    c->gd.nzm = c->nzm;
+    c->gd.nrm = c->nrm;
+    c->gd.nr = c->nr;
+    c->gd.nth2 = c->nth2;
+    c->gd.P = c->P;
+  }
+  // masks (SlipBCs at time 0)
+  TRY(dev_alloc(c, (void **)&c->mask, (size_t)2 * c->P));
+  tc::slip_bcs(c->p, c->g, 0.0, c->icyl, c->ocyl);
+  TRY(upload_masks(c));
+  // LU tables
+  c->lu_nt = 6 * c->nr;
+  {
+    const size_t n = (size_t)3 * 3 * 4 * c->lu_nt;
+    TRY(dev_alloc(c, (void **)&c->lu, sizeof(double) * n));
+    HTRY(hipHostMalloc((void **)&c->lu_host, sizeof(double) * n, hipHostMallocDefault));
+    HTRY(hipEventCreateWithFlags(&c->lu_copied, hipEventDisableTiming));
+    HTRY(hipEventRecord(c->lu_copied, c->stream));
+  }
+  // twiddles
+  {
+    std::vector<double> tw;
+    tc::fill_twiddles(c->nthm, tw);
+    TRY(dev_alloc(c, (void **)&c->tw_t, sizeof(double) * tw.size()));
+    HTRY(hipMemcpy(c->tw_t, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    tc::fill_twiddles(c->nzm, tw);
+    TRY(dev_alloc(c, (void **)&c->tw_z, sizeof(double) * tw.size()));
+    HTRY(hipMemcpy(c->tw_z, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  }
+  // reductions
+  TRY(dev_alloc(c, (void **)&c->red, sizeof(double) * 1024 * 8));
+  TRY(dev_alloc(c, (void **)&c->red_out, sizeof(double) * 16));
+  HTRY(hipHostMalloc((void **)&c->red_host, sizeof(double) * 16, hipHostMallocDefault));
+  set_substep(c, 1);
+  TRY(refresh_tables(c));
+  HTRY(hipStreamSynchronize(c->stream));
+#undef TRY
+#undef HTRY
+  *out = c;
+  return 0;
+}
+
+int tc_destroy(tc_ctx *c) {
+  destroy_ctx(c);
+  return 0;
+}
+
+static int check_field(tc_ctx *c, int field) {
+  if (field < 0 || field >= TC_NFIELDS) return fail("field id out of range");
+  return 0;
+}
+
+int tc_upload_field(tc_ctx *c, int field, const double *host, int layout) {
+  CTXCHK(c);
+  if (check_field(c, field)) return -1;
+  if (!host) return fail("null host pointer");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (layout == TC_LAYOUT_DEVICE) {
+    HIPCHK(hipMemcpy(c->f[field], host, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice));
+    return 0;
+  }
+  if (layout != TC_LAYOUT_FORTRAN) return fail("unknown layout");
+  std::vector<double> d((size_t)c->N);
+  const int nr = c->nr, nzm = c->nzm, nthm = c->nthm;
+  for (int i = 0; i < nthm; i++)
+    for (int j = 0; j < nzm; j++) {
+      const double *src = host + ((size_t)j + (size_t)nzm * i) * nr;
+      for (int k = 0; k < nr; k++) d[(size_t)k * c->P + (size_t)j * nthm + i] = src[k];
+    }
+  HIPCHK(hipMemcpy(c->f[field], d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int tc_download_field(tc_ctx *c, int field, double *host, int layout) {
+  CTXCHK(c);
+  if (check_field(c, field)) return -1;
+  if (!host) return fail("null host pointer");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (layout == TC_LAYOUT_DEVICE) {
+    HIPCHK(hipMemcpy(host, c->f[field], sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost));
+    return 0;
+  }
+  if (layout != TC_LAYOUT_FORTRAN) return fail("unknown layout");
+  std::vector<double> d((size_t)c->N);
+  HIPCHK(hipMemcpy(d.data(), c->f[field], sizeof(double) * d.size(), hipMemcpyDeviceToHost));
+  const int nr = c->nr, nzm = c->nzm, nthm = c->nthm;
+  for (int i = 0; i < nthm; i++)
+    for (int j = 0; j < nzm; j++) {
+      double *dst = host + ((size_t)j + (size_t)nzm * i) * nr;
+      for (int k = 0; k < nr; k++) dst[k] = d[(size_t)k * c->P + (size_t)j * nthm + i];
+    }
+  return 0;
+}
+
+int tc_set_masks(tc_ctx *c, const int *icylsl, const int *ocylsl) {
+  CTXCHK(c);
+  if (!icylsl || !ocylsl) return fail("null mask pointer");
+  const size_t n = (size_t)c->P;
+  for (size_t q = 0; q < n; q++)
+    if ((icylsl[q] != 0 && icylsl[q] != 1) || (ocylsl[q] != 0 && ocylsl[q] != 1))
+      return fail("masks must hold 0 or 1");
+  c->icyl.assign(icylsl, icylsl + n);
+  c->ocyl.assign(ocylsl, ocylsl + n);
+  return upload_masks(c);
+}
+
+int tc_get_masks(tc_ctx *c, int *icylsl, int *ocylsl) {
+  CTXCHK(c);
+  std::copy(c->icyl.begin(), c->icyl.end(), icylsl);
+  std::copy(c->ocyl.begin(), c->ocyl.end(), ocylsl);
+  return 0;
+}
+
+int tc_set_time(tc_ctx *c, double time, double dt) {
+  CTXCHK(c);
+  c->time = time;
+  c->dt = dt;
+  return 0;
+}
+
+int tc_get_scalars(tc_ctx *c, double *o) {
+  CTXCHK(c);
+  o[0] = c->time; o[1] = c->dt; o[2] = c->beta; o[3] = c->al;
+  o[4] = c->ga; o[5] = c->ro; o[6] = c->vinner_t; o[7] = c->dvinner_t;
+  return 0;
+}
+
+int tc_begin_step(tc_ctx *c) {
+  CTXCHK(c);
+  return begin_step(c);
+}
+
+int tc_set_substep(tc_ctx *c, int ns) {
+  CTXCHK(c);
+  if (ns < 1 || ns > c->p.nsst) return fail("substep out of range");
+  set_substep(c, ns);
+  return 0;
+}
+
+int tc_set_fused(tc_ctx *c, int fused) {
+  CTXCHK(c);
+  c->fused = fused != 0;
+  return 0;
+}
+
+#define ROUTINE(name, body) \
+  int name(tc_ctx *c) {     \
+    CTXCHK(c);              \
+    body;                   \
+    return launch_status(); \
+  }
+
+ROUTINE(tc_explicit_terms_qt, tc::launch_explicit_qt(c))
+ROUTINE(tc_explicit_terms_qr, tc::launch_explicit_qr(c))
+ROUTINE(tc_explicit_terms_qz, tc::launch_explicit_qz(c))
+ROUTINE(tc_implicit_and_update_qt, tc::launch_implicit_qt(c))
+ROUTINE(tc_implicit_and_update_qr, tc::launch_implicit_qr(c))
+ROUTINE(tc_implicit_and_update_qz, tc::launch_implicit_qz(c))
+ROUTINE(tc_calc_local_divergence, tc::launch_divergence(c, 1.0 / (c->al * c->dt)))
+ROUTINE(tc_solve_pressure_correction, tc::launch_poisson(c))
+ROUTINE(tc_correct_velocity, tc::launch_correct_velocity(c, c->al))
+ROUTINE(tc_correct_pressure, tc::launch_correct_pressure(c))
+
+int tc_time_marcher(tc_ctx *c) {
+  CTXCHK(c);
+  int rc = begin_step(c);
+  if (rc) return rc;
+  for (int ns = 1; ns <= c->p.nsst; ns++) {
+    set_substep(c, ns);
+    rc = run_substep(c);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int tc_project(tc_ctx *c) {
+  CTXCHK(c);
+  const double al = c->al, dt = c->dt;
+  c->al = 1.0;
+  c->dt = 1.0;
+  tc::launch_divergence(c, 1.0);
+  tc::launch_poisson(c);
+  tc::launch_correct_velocity(c, 1.0);
+  c->al = al;
+  c->dt = dt;
+  int rc = launch_status();
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tc_calc_max_cfl(tc_ctx *c, double *cflm) {
+  CTXCHK(c);
+  tc::launch_max_cfl(c);
+  int rc = launch_status();
+  if (rc) return rc;
+  double v;
+  rc = read_reduction(c, 1, &v);
+  if (rc) return rc;
+  *cflm = std::max(0.00000001, v);
+  return 0;
+}
+
+int tc_check_divergence(tc_ctx *c, double *qmax) {
+  CTXCHK(c);
+  tc::launch_max_div(c);
+  int rc = launch_status();
+  if (rc) return rc;
+  return read_reduction(c, 1, qmax);
+}
+
+int tc_calc_torque(tc_ctx *c, double *inner, double *outer) {
+  CTXCHK(c);
+  tc::launch_torque(c);
+  int rc = launch_status();
+  if (rc) return rc;
+  double s[2];
+  rc = read_reduction(c, 2, s);
+  if (rc) return rc;
+  // CalcTorque.F90:121-150
+  const tc_params &p = c->p;
+  const double reninv = 1.0 / p.ren;
+  const double jlam = 2.0 * p.rint * p.rint * p.rext * p.rext * (p.virotint / p.rint - p.virotext / p.rext) /
+                      (p.rext * p.rext - p.rint * p.rint) * reninv;
+  const double jlami = 1.0 / jlam;
+  const double rc2 = c->g.rc[2], rcn = c->g.rc[c->nrm];
+  const double lo = s[0] * (rc2 * rc2 * rc2) * reninv * jlami;
+  const double up = s[1] * (rcn * rcn * rcn) * reninv * jlami;
+  const double my_surface = 1.0 / (c->nthm * c->nzm);
+  *inner = lo * my_surface;
+  *outer = up * my_surface;
+  return 0;
+}
+
+int tc_calc_velocity_stats(tc_ctx *c, double *vmax3, double *rms3) {
+  CTXCHK(c);
+  tc::launch_velocity_stats(c);
+  int rc = launch_status();
+  if (rc) return rc;
+  double v[6];
+  rc = read_reduction(c, 6, v);
+  if (rc) return rc;
+  const double volt = c->g.pi * (c->p.rext * c->p.rext - c->p.rint * c->p.rint);
+  const double den = (double)c->nthm * (double)c->nzm * volt;
+  for (int q = 0; q < 3; q++) {
+    vmax3[q] = v[q];
+    rms3[q] = v[3 + q] / den;
+  }
+  return 0;
+}
+
+int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
+  CTXCHK(c);
+  // main.F90:156-223
+  int errorcode = 0;
+  double instcfl;
+  int rc = tc_calc_max_cfl(c, &instcfl);
+  if (rc) return rc;
+  const tc_params &p = c->p;
+  if (p.idtv != 0) {
+    if (ntime > 1) {
+      c->dt = (instcfl < 1.0e-8) ? p.dtmax : p.limitcfl / instcfl;
+      if (c->dt > p.dtmax) c->dt = p.dtmax;
+    } else {
+      c->dt = p.dtmin;
+    }
+    if (c->dt < p.dtmin) errorcode = 166;
+  } else {
+    c->dt = p.dtmax;
+    instcfl = instcfl * c->dt;
+    if (instcfl > p.limitcfl) errorcode = 165;
+  }
+  double out[9] = {instcfl, c->dt, 0, 0, 0, 0, 0, 0, 0};
+  rc = tc_time_marcher(c);
+  if (rc) return rc;
+  c->time = c->time + c->dt;
+  out[2] = c->time;
+  if (ntime == 1 || std::fmod(c->time, p.tout) < c->dt) {
+    double vmax[3], rms[3];
+    rc = tc_calc_velocity_stats(c, vmax, rms);
+    if (rc) return rc;
+    if (vmax[0] > p.limitvel && vmax[1] > p.limitvel) errorcode = 266;
+    rc = tc_calc_max_cfl(c, &out[0]);
+    if (rc) return rc;
+    rc = tc_check_divergence(c, &out[3]);
+    if (rc) return rc;
+    rc = tc_calc_torque(c, &out[4], &out[5]);
+    if (rc) return rc;
+    out[6] = vmax[0];
+    out[7] = vmax[1];
+    out[8] = vmax[2];
+    if (std::fabs(out[3]) > p.resid) errorcode = 169;
+  }
+  if (c->time > p.tmax) errorcode = 333;
+  if (walltime > p.walltimemax) errorcode = 334;
+  if (ntime == p.ntst) errorcode = 555;
+  if (o)
+    for (int q = 0; q < 9; q++) o[q] = out[q];
+  return errorcode;
+}
+
+int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed, double amp, double *qt,
+                              double *qr, double *qz) {
+  if (!p || !qt || !qr || !qz) return fail("null argument");
+  tc::Grid g;
+  std::string e = tc::build_grid(*p, g);
+  if (!e.empty()) return fail(e);
+  const int nr = g.nr, nrm = g.nrm;
+  const size_t n = (size_t)nr * p->nzm * p->nthm;
+  unsigned long long state = seed;
+  auto next = [&state]() {
+    unsigned long long z = (state += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  // CreateInitialConditions.F90:180-182 laminar profile coefficients
+  const double eta = p->rint / p->rext;
+  const double alam = (p->virotext - eta * eta * p->virotint) / (1 - eta * eta);
+  const double blam = (p->virotint - p->virotext) * p->rint * p->rint / (1 - eta * eta);
+  for (size_t q = 0; q < n; q++) {
+    const int k = (int)(q % nr) + 1;
+    qt[q] = 0.0;
+    if (k <= nrm) qt[q] = alam * g.rm[k] + blam / g.rm[k] + amp * (2.0 * next() - 1.0);
+  }
+  for (size_t q = 0; q < n; q++) {
+    const int k = (int)(q % nr) + 1;
+    qr[q] = 0.0;
+    if (k >= 2 && k <= nrm) qr[q] = amp * (2.0 * next() - 1.0);
+  }
+  for (size_t q = 0; q < n; q++) {
+    const int k = (int)(q % nr) + 1;
+    qz[q] = 0.0;
+    if (k <= nrm) qz[q] = amp * (2.0 * next() - 1.0);
+  }
+  return 0;
+}
+
+int tc_synchronize(tc_ctx *c) {
+  CTXCHK(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tc_kernel_timing(tc_ctx *c, int enable) {
+  CTXCHK(c);
+  c->timing = enable != 0;
+  return 0;
+}
+
+int tc_kernel_timing_read(tc_ctx *c, const char *kernel, double *total_ms, long long *count,
+                          int reset) {
+  CTXCHK(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto &t : c->timers) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+    auto &acc = c->timing_acc[t.name];
+    acc.first += ms;
+    acc.second += 1;
+    c->event_pool.push_back(t.a);
+    c->event_pool.push_back(t.b);
+  }
+  c->timers.clear();
+  auto it = c->timing_acc.find(kernel ? kernel : "");
+  if (total_ms) *total_ms = (it == c->timing_acc.end()) ? 0.0 : it->second.first;
+  if (count) *count = (it == c->timing_acc.end()) ? 0 : it->second.second;
+  if (reset) c->timing_acc.clear();
+  return 0;
+}
+
+int tc_device_bytes(tc_ctx *c, unsigned long long *bytes) {
+  CTXCHK(c);
+  *bytes = c->device_bytes;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Internal definitions shared by the MI355X kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tc_abi.h"
+#include "tc_grid.h"
+
+namespace tc {
+
+// Device view of the grid: pointers into one device buffer of 1-based
+// vectors (v[k] == reference v(k)) plus the scalars the kernels need.
+struct GridDev {
+  const double *rc, *rm, *usrc, *usrm, *udx2rm, *udx2m, *udx2c, *rudx2c, *udx1vis1,
+      *udx1vis2, *udx1qm, *pvol;
+  const double *ap1j, *am1j, *ac1j, *ap2je, *am2je, *ac2je, *ap3j, *am3j, *ac3j;
+  const double *apph, *amph, *acph, *ak1, *ak3;
+  double dxt, dxz, dxtq, dxzq, ren, romeg;
+  double ap1j_fs2, ac1j_fs1, ac1j_fs2, am1j_fs1, ap3j_fs2, ac3j_fs1, ac3j_fs2, am3j_fs1;
+  int nthm, nzm, nrm, nr, nth2;
+  long long P;  // points per r-plane = nzm*nthm
+};
+
+// Radix plan of one FFT length.
+struct FFTPlan {
+  int n = 0;
+  int nst = 0;
+  int radix[24] = {0};
+  int lines = 1;  // FFT lines per 256-thread block
+};
+
+// Helmholtz LU tables: per (substep slot, component, mask variant) a block of
+// 6*NT doubles: dl | du | du2 | rd=1/d | row scale | pivot flag.
+enum { LU_QT = 0, LU_QR = 1, LU_QZ = 2 };
+
+struct Timer {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace tc
+
+struct tc_ctx {
+  tc_params p;
+  tc::Grid g;
+  int nthm = 0, nzm = 0, nrm = 0, nr = 0, nth2 = 0;
+  long long P = 0, N = 0;  // plane size, field size (nr planes)
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double *f[TC_NFIELDS] = {nullptr};
+  double2 *spec = nullptr;   // nrm x nzm x nth2 complex, [r][kz][ktheta]
+  double *cpw = nullptr;     // nrm x nzm x nth2 real: Thomas multipliers
+  double *gvbuf = nullptr;   // device grid vectors
+  tc::GridDev gd;
+  unsigned char *mask = nullptr;  // device [2][nzm*nthm] (icyl, ocyl), [j][i]
+  std::vector<int> icyl, ocyl;    // host copies, (nzm,nthm) column-major
+  bool masks_uniform_one = true;
+  // Helmholtz LU tables
+  double *lu = nullptr;        // device
+  double *lu_host = nullptr;   // pinned staging
+  hipEvent_t lu_copied = nullptr;
+  int lu_nt = 0;               // doubles per table = 6*(nrm+1)
+  double lu_key[3] = {-1, -1, -1};
+  bool lu_valid = false;
+  // FFT
+  tc::FFTPlan plan_t, plan_z;
+  double2 *tw_t = nullptr, *tw_z = nullptr;
+  // reductions
+  double *red = nullptr;        // device partials
+  double *red_out = nullptr;    // device results (16)
+  double *red_host = nullptr;   // pinned copy of red_out
+  int red_blocks = 0;
+  // scalars of module param
+  double time = 0, dt = 0, beta = 0, al = 0, ga = 0, ro = 0, vinner_t = 0, dvinner_t = 0;
+  int ns = 1;
+  bool fused = true;
+  // timing
+  bool timing = false;
+  std::vector<tc::Timer> timers;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, long long>> timing_acc;
+  unsigned long long device_bytes = 0;
+};
+
+namespace tc {
+
+// kernel launchers (tc_kernels.hip)
+void launch_explicit_qt(tc_ctx *c);
+void launch_explicit_qr(tc_ctx *c);
+void launch_explicit_qz(tc_ctx *c);
+void launch_explicit_all(tc_ctx *c);
+void launch_implicit_qt(tc_ctx *c);
+void launch_implicit_qr(tc_ctx *c);
+void launch_implicit_qz(tc_ctx *c);
+void launch_divergence(tc_ctx *c, double usdt);
+void launch_correct_velocity(tc_ctx *c, double aldt);
+void launch_correct_pressure(tc_ctx *c);
+void launch_correct_both(tc_ctx *c);
+void launch_zero_wall_qr(tc_ctx *c);
+// reductions: results land in c->red_out and are copied to c->red_host
+void launch_max_cfl(tc_ctx *c);
+void launch_max_div(tc_ctx *c);
+void launch_torque(tc_ctx *c);
+void launch_velocity_stats(tc_ctx *c);
+
+// Poisson solve (tc_poisson.hip)
+std::string make_fft_plan(int n, FFTPlan &pl, int max_lines);
+void fill_twiddles(int n, std::vector<double> &tw);
+void launch_poisson(tc_ctx *c);
+
+// timing helpers (tc_abi.cpp)
+void timer_begin(tc_ctx *c, const char *name);
+void timer_end(tc_ctx *c);
+
+}  // namespace tc

@@ -1,0 +1,366 @@
+// Pressure-Poisson solve of SolvePressureCorrection.F90:93-208 on one MI355X.
+//
+// The reference transforms z (r2c) then theta (c2c), solves one complex
+// radial tridiagonal per (kz, ktheta) and transforms back, with eight pencil
+// transposes in between.  Here the field never leaves its [r][z][theta]
+// layout:
+//   1. r2c along theta (contiguous rows; two real rows packed into one
+//      complex FFT), giving spec[r][z][ktheta], ktheta = 0..nthm/2;
+//   2. c2c along z on tiles of ktheta columns staged in LDS;
+//   3. Thomas along r per (kz, ktheta) mode, lanes across ktheta, with the
+//      1/(nthm*nzm) normalisation folded into the right-hand side;
+//   4. inverse c2c along z, then c2r along theta back into dph.
+// The 2-D DFT is separable, so the order of the two directions only changes
+// rounding.  The FFTs are Stockham autosort passes in LDS with radix 8/4/2/3/5
+// butterflies and twiddles from a host-computed table.
+#include <cmath>
+
+#include "tc_internal.h"
+
+namespace tc {
+
+#define DEVI __device__ __forceinline__
+
+DEVI double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+DEVI double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+DEVI double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by (s * i), s = +-1
+DEVI double2 cmul_si(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
+
+// In-register DFT of length R with exponent sign sg (-1 forward, +1 backward).
+template <int R>
+DEVI void dft(double2 *v, double sg);
+
+template <>
+DEVI void dft<2>(double2 *v, double) {
+  const double2 a = cadd(v[0], v[1]), b = csub(v[0], v[1]);
+  v[0] = a;
+  v[1] = b;
+}
+
+template <>
+DEVI void dft<3>(double2 *v, double sg) {
+  const double s60 = 0.86602540378443864676;
+  const double2 t1 = cadd(v[1], v[2]);
+  const double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+  const double2 d = csub(v[1], v[2]);
+  const double2 t3 = make_double2(d.x * s60, d.y * s60);
+  const double2 it3 = cmul_si(t3, sg);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, it3);
+  v[2] = csub(t2, it3);
+}
+
+template <>
+DEVI void dft<4>(double2 *v, double sg) {
+  const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  const double2 b0 = cadd(v[1], v[3]), b1 = cmul_si(csub(v[1], v[3]), sg);
+  v[0] = cadd(a0, b0);
+  v[2] = csub(a0, b0);
+  v[1] = cadd(a1, b1);
+  v[3] = csub(a1, b1);
+}
+
+template <>
+DEVI void dft<5>(double2 *v, double sg) {
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+  const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const double2 x0 = cadd(v[0], cadd(t1, t2));
+  const double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  const double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  const double2 b1 = cmul_si(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sg);
+  const double2 b2 = cmul_si(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sg);
+  v[0] = x0;
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+template <>
+DEVI void dft<8>(double2 *v, double sg) {
+  double2 e[4] = {v[0], v[2], v[4], v[6]};
+  double2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e, sg);
+  dft<4>(o, sg);
+  const double c = 0.70710678118654752440;
+  // o[q] *= exp(sg*2*pi*i*q/8)
+  o[1] = make_double2(c * (o[1].x - sg * o[1].y), c * (o[1].y + sg * o[1].x));
+  o[2] = cmul_si(o[2], sg);
+  o[3] = make_double2(c * (-o[3].x - sg * o[3].y), c * (-o[3].y + sg * o[3].x));
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    v[q] = cadd(e[q], o[q]);
+    v[q + 4] = csub(e[q], o[q]);
+  }
+}
+
+template <int R>
+constexpr int maxb() {
+  return (16 / R) > 0 ? 16 / R : 1;
+}
+
+// One Stockham pass of radix R over a line of n points in LDS (in place:
+// read all inputs to registers, barrier, write, barrier).  Ns is the product
+// of the radices of the previous passes.  All threads of the block must call.
+template <int R>
+DEVI void fft_pass(double2 *buf, int n, int Ns, int t, int T, const double2 *__restrict__ tw,
+                   double sg) {
+  constexpr int MB = maxb<R>();
+  const int nb = n / R;
+  const int twstep = n / (Ns * R);
+  double2 v[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; q++) {
+    const int b = t + q * T;
+    if (b < nb) {
+      const int k = b % Ns;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        double2 x = buf[b + r * nb];
+        if (r > 0 && k > 0) {
+          double2 w = tw[r * k * twstep];
+          w.y = (sg < 0) ? w.y : -w.y;
+          x = cmul(x, w);
+        }
+        v[q][r] = x;
+      }
+      dft<R>(v[q], sg);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MB; q++) {
+    const int b = t + q * T;
+    if (b < nb) {
+      const int k = b % Ns;
+      const int base = (b / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[base + r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+DEVI void fft_line(double2 *buf, const FFTPlan &pl, int t, int T, const double2 *__restrict__ tw,
+                   double sg) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nst; s++) {
+    const int R = pl.radix[s];
+    switch (R) {
+      case 8: fft_pass<8>(buf, pl.n, Ns, t, T, tw, sg); break;
+      case 4: fft_pass<4>(buf, pl.n, Ns, t, T, tw, sg); break;
+      case 2: fft_pass<2>(buf, pl.n, Ns, t, T, tw, sg); break;
+      case 3: fft_pass<3>(buf, pl.n, Ns, t, T, tw, sg); break;
+      default: fft_pass<5>(buf, pl.n, Ns, t, T, tw, sg); break;
+    }
+    Ns *= R;
+  }
+}
+
+// r2c along theta for row pairs.  row r of the real input is at in + r*nthm,
+// row r of the half spectrum at spec + r*nth2.
+__global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict__ in,
+                                                       double2 *__restrict__ spec, FFTPlan pl,
+                                                       const double2 *__restrict__ tw,
+                                                       long long nrows, int nth2) {
+  extern __shared__ double2 lds[];
+  const int n = pl.n, L = pl.lines, T = 256 / L;
+  const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
+  const long long pair = (long long)blockIdx.x * L + ln;
+  const long long r0 = 2 * pair, r1 = r0 + 1;
+  const bool v0 = r0 < nrows, v1 = r1 < nrows;
+  double2 *buf = lds + ln * (n + 1);
+  const double *a = in + r0 * n, *b = in + r1 * n;
+  for (int e = t; e < n; e += T) buf[e] = make_double2(v0 ? a[e] : 0.0, v1 ? b[e] : 0.0);
+  __syncthreads();
+  fft_line(buf, pl, t, T, tw, -1.0);
+  for (int k = t; k < nth2; k += T) {
+    const double2 z = buf[k];
+    const double2 zc = buf[(n - k) % n];
+    // A = (Z + conj(Zc))/2, B = (Z - conj(Zc))/(2i)
+    const double2 A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
+    const double2 B = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+    if (v0) spec[r0 * nth2 + k] = A;
+    if (v1) spec[r1 * nth2 + k] = B;
+  }
+}
+
+// c2r along theta for row pairs; the imaginary parts of ktheta = 0 and
+// (n even) ktheta = n/2 are ignored, as a c2r transform does.
+__global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict__ spec,
+                                                       double *__restrict__ out, FFTPlan pl,
+                                                       const double2 *__restrict__ tw,
+                                                       long long nrows, int nth2) {
+  extern __shared__ double2 lds[];
+  const int n = pl.n, L = pl.lines, T = 256 / L;
+  const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
+  const long long pair = (long long)blockIdx.x * L + ln;
+  const long long r0 = 2 * pair, r1 = r0 + 1;
+  const bool v0 = r0 < nrows, v1 = r1 < nrows;
+  double2 *buf = lds + ln * (n + 1);
+  for (int k = t; k < n; k += T) {
+    const bool lo = (k < nth2);
+    const int kk = lo ? k : n - k;
+    double2 A = v0 ? spec[r0 * nth2 + kk] : make_double2(0.0, 0.0);
+    double2 B = v1 ? spec[r1 * nth2 + kk] : make_double2(0.0, 0.0);
+    if (!lo) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    if (k == 0 || 2 * k == n) {
+      A.y = 0.0;
+      B.y = 0.0;
+    }
+    buf[k] = make_double2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  fft_line(buf, pl, t, T, tw, 1.0);
+  for (int e = t; e < n; e += T) {
+    const double2 z = buf[e];
+    if (v0) out[r0 * n + e] = z.x;
+    if (v1) out[r1 * n + e] = z.y;
+  }
+}
+
+// c2c along z for a tile of pl.lines ktheta columns of one r-plane.
+__global__ __launch_bounds__(256) void k_fft_z(double2 *__restrict__ spec, FFTPlan pl,
+                                               const double2 *__restrict__ tw, int nth2,
+                                               double sg) {
+  extern __shared__ double2 lds[];
+  const int n = pl.n, L = pl.lines, T = 256 / L;
+  const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
+  const int c0 = blockIdx.x * L;
+  double2 *plane = spec + (long long)blockIdx.y * n * nth2;
+  for (int e = threadIdx.x; e < L * n; e += 256) {
+    const int z = e / L, c = e - z * L;
+    lds[c * (n + 1) + z] = (c0 + c < nth2) ? plane[(long long)z * nth2 + c0 + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_line(lds + ln * (n + 1), pl, t, T, tw, sg);
+  for (int e = threadIdx.x; e < L * n; e += 256) {
+    const int z = e / L, c = e - z * L;
+    if (c0 + c < nth2) plane[(long long)z * nth2 + c0 + c] = lds[c * (n + 1) + z];
+  }
+}
+
+// SolvePressureCorrection.F90:171-192: radial tridiagonal per (kz, ktheta)
+// mode, sub-diagonal amph, super-diagonal apph and diagonal
+// acph - ak3(kz) - ak1(ktheta)*usrm^2.  Thomas without pivoting (the rows are
+// diagonally dominant).  The singular (0,0) Neumann mode is solved on rows
+// 1..nrm-1 with phi(nrm) = 0.
+__global__ __launch_bounds__(256) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
+                                                         double *__restrict__ cpw,
+                                                         double inv_norm) {
+  const long long M = (long long)g.nzm * g.nth2;
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int kz = (int)(m / g.nth2), kt = (int)(m - (long long)kz * g.nth2);
+  const double sz = g.ak3[kz + 1], st = g.ak1[kt + 1];
+  const int nn = (m == 0) ? g.nrm - 1 : g.nrm;
+  // forward elimination
+  double b = g.acph[1] - sz - st * (g.usrm[1] * g.usrm[1]);
+  double rden = 1.0 / b;
+  double cp = g.apph[1] * rden;
+  double2 d = spec[m];
+  double2 dp = make_double2(d.x * inv_norm * rden, d.y * inv_norm * rden);
+  cpw[m] = cp;
+  spec[m] = dp;
+  for (int k = 2; k <= nn; k++) {
+    const long long o = (long long)(k - 1) * M + m;
+    const double a = g.amph[k];
+    b = g.acph[k] - sz - st * (g.usrm[k] * g.usrm[k]);
+    rden = 1.0 / (b - a * cp);
+    d = spec[o];
+    dp = make_double2((d.x * inv_norm - a * dp.x) * rden, (d.y * inv_norm - a * dp.y) * rden);
+    spec[o] = dp;
+    if (k < nn) {
+      cp = g.apph[k] * rden;
+      cpw[o] = cp;
+    }
+  }
+  // back substitution
+  double2 x = dp;
+  for (int k = nn - 1; k >= 1; k--) {
+    const long long o = (long long)(k - 1) * M + m;
+    const double c = cpw[o];
+    const double2 y = spec[o];
+    x = make_double2(y.x - c * x.x, y.y - c * x.y);
+    spec[o] = x;
+  }
+  if (m == 0) spec[(long long)(g.nrm - 1) * M] = make_double2(0.0, 0.0);
+}
+
+std::string make_fft_plan(int n, FFTPlan &pl, int max_lines) {
+  pl = FFTPlan();
+  pl.n = n;
+  int m = n;
+  while (m % 8 == 0 && m > 1) { pl.radix[pl.nst++] = 8; m /= 8; }
+  while (m % 4 == 0 && m > 1) { pl.radix[pl.nst++] = 4; m /= 4; }
+  while (m % 2 == 0 && m > 1) { pl.radix[pl.nst++] = 2; m /= 2; }
+  while (m % 3 == 0 && m > 1) { pl.radix[pl.nst++] = 3; m /= 3; }
+  while (m % 5 == 0 && m > 1) { pl.radix[pl.nst++] = 5; m /= 5; }
+  if (m != 1) return "FFT length " + std::to_string(n) + " has a prime factor other than 2, 3, 5";
+  // lines per 256-thread block: every pass must fit maxb(R) butterflies per
+  // thread, and the LDS tile (lines*(n+1) complex) must fit 64 KiB
+  int L = max_lines;
+  auto fits = [&](int lines) {
+    const int T = 256 / lines;
+    for (int s = 0; s < pl.nst; s++) {
+      const int R = pl.radix[s];
+      const int mb = (16 / R) > 0 ? 16 / R : 1;
+      if (n / R > mb * T) return false;
+    }
+    return (long long)lines * (n + 1) * 16 <= 65536;
+  };
+  while (L > 1 && !fits(L)) L /= 2;
+  if (!fits(L)) return "FFT length " + std::to_string(n) + " is too long for one block";
+  pl.lines = L;
+  return std::string();
+}
+
+void fill_twiddles(int n, std::vector<double> &tw) {
+  tw.resize(2 * (size_t)n);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < n; m++) {
+    const long double a = twopi * (long double)m / (long double)n;
+    tw[2 * (size_t)m] = (double)cosl(a);
+    tw[2 * (size_t)m + 1] = (double)(-sinl(a));
+  }
+}
+
+void launch_poisson(tc_ctx *c) {
+  const long long nrows = (long long)c->nrm * c->nzm;
+  const long long npairs = (nrows + 1) / 2;
+  const FFTPlan &pt = c->plan_t, &pz = c->plan_z;
+  const size_t lds_t = (size_t)pt.lines * (pt.n + 1) * sizeof(double2);
+  const size_t lds_z = (size_t)pz.lines * (pz.n + 1) * sizeof(double2);
+  const unsigned bt = (unsigned)((npairs + pt.lines - 1) / pt.lines);
+  const dim3 gz((unsigned)((c->nth2 + pz.lines - 1) / pz.lines), (unsigned)c->nrm);
+  const long long M = (long long)c->nzm * c->nth2;
+
+  timer_begin(c, "fft_theta_fwd");
+  hipLaunchKernelGGL(k_fft_theta_fwd, dim3(bt), dim3(256), lds_t, c->stream, c->f[TC_DPH], c->spec,
+                     pt, c->tw_t, nrows, c->nth2);
+  timer_end(c);
+  timer_begin(c, "fft_z_fwd");
+  hipLaunchKernelGGL(k_fft_z, gz, dim3(256), lds_z, c->stream, c->spec, pz, c->tw_z, c->nth2, -1.0);
+  timer_end(c);
+  timer_begin(c, "poisson_tridiag");
+  hipLaunchKernelGGL(k_poisson_tridiag, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
+                     c->gd, c->spec, c->cpw, 1.0 / ((double)c->nthm * (double)c->nzm));
+  timer_end(c);
+  timer_begin(c, "fft_z_inv");
+  hipLaunchKernelGGL(k_fft_z, gz, dim3(256), lds_z, c->stream, c->spec, pz, c->tw_z, c->nth2, 1.0);
+  timer_end(c);
+  timer_begin(c, "fft_theta_inv");
+  hipLaunchKernelGGL(k_fft_theta_inv, dim3(bt), dim3(256), lds_t, c->stream, c->spec, c->f[TC_DPH],
+                     pt, c->tw_t, nrows, c->nth2);
+  timer_end(c);
+}
+
+}  // namespace tc

@@ -1,0 +1,65 @@
+"""Shared set-up for the parity tests: identical states on the device and in
+the oracle, and the comparison metrics."""
+import numpy as np
+
+
+def small_params(tc, nthm=32, nzm=24, nrm=33, **kw):
+    p = tc.configs.small_64()
+    p.update(nthm=nthm, nzm=nzm, nrm=nrm)
+    p.update(kw)
+    return p
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b.ravel())
+    num = np.linalg.norm((a - b).ravel())
+    return num / den if den > 0 else num
+
+
+def random_state(p, seed, amp=1e-2, pressure=True, history=True):
+    """Couette profile plus noise on every field the hot path reads."""
+    rng = np.random.default_rng(seed)
+    nr, nzm, nthm, nrm = p["nrm"] + 1, p["nzm"], p["nthm"], p["nrm"]
+    shape = (nr, nzm, nthm)
+    st = {}
+    rint, rext = p["rint"], p["rext"]
+    eta = rint / rext
+    vi, ve = p["virotint"], p["virotext"]
+    A = (ve - eta * eta * vi) / (1 - eta * eta)
+    B = (vi - ve) * rint * rint / (1 - eta * eta)
+    r = np.linspace(rint, rext, nr)
+    rm = 0.5 * (r[1:] + r[:-1])
+    qt = np.zeros(shape, order="F")
+    qt[:nrm] = (A * rm + B / rm)[:, None, None] + amp * rng.uniform(-1, 1, (nrm, nzm, nthm))
+    qr = np.zeros(shape, order="F")
+    qr[1:nrm] = amp * rng.uniform(-1, 1, (nrm - 1, nzm, nthm))
+    qz = np.zeros(shape, order="F")
+    qz[:nrm] = amp * rng.uniform(-1, 1, (nrm, nzm, nthm))
+    st.update(qt=qt, qr=qr, qz=qz)
+    for name in ["pr", "dph", "ru1", "ru2", "ru3", "dq", "qcap"]:
+        a = np.zeros(shape, order="F")
+        if (name == "pr" and pressure) or (name.startswith("ru") and history) or name in ("dph", "dq", "qcap"):
+            a[:nrm] = amp * rng.uniform(-1, 1, (nrm, nzm, nthm))
+        st[name] = a
+    return st
+
+
+def load_both(dev, orc, st):
+    for k, v in st.items():
+        dev.set(k, v)
+        orc.set(k, v)
+
+
+def interior(a, nrm, kind="cell"):
+    """The planes the reference defines for a field: 1..nrm for cell fields,
+    1..nr for qr."""
+    return a if kind == "qr" else a[:nrm]
+
+
+def pressure_modulo_gauge(pr, ref):
+    """pr - ref after removing the theta-z mean profile difference, which the
+    singular (0,0) Poisson mode leaves undetermined in the reference."""
+    d = pr - ref
+    return d - d.mean(axis=(1, 2), keepdims=True)

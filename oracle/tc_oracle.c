@@ -1188,81 +1188,105 @@ void or_calc_local_divergence(or_state *s) {
 /* ------------------------------------------------------------------ */
 /* FFT: unnormalised DFT X[k] = sum_n x[n] exp(sign*2*pi*i*n*k/N)      */
 /* (the transform FFTW computes, SolvePressureCorrection.F90:105-139)  */
-/* recursive mixed-radix decimation in time                           */
+/* recursive mixed-radix decimation in time over the prime factors of  */
+/* N, twiddles from a per-length table exp(-2 pi i m/N)                */
 /* ------------------------------------------------------------------ */
-static void twiddle(int m, int n, int sign, double *w) {
-  /* exp(sign*2*pi*i*m/n), argument reduced to [0, n) */
-  m %= n;
-  if (m < 0) m += n;
-  long double a = (long double)2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
-  w[0] = (double)cosl(a);
-  w[1] = (double)(sign * sinl(a));
+typedef struct {
+  int n, nf, f[64];
+  double *tw; /* 2n: cos, -sin of 2 pi m / n */
+} fft_plan;
+
+#define OR_MAX_PLANS 32
+static fft_plan g_plans[OR_MAX_PLANS];
+static int g_nplans = 0;
+
+static const fft_plan *fft_get_plan(int n) {
+  const fft_plan *r = NULL;
+#pragma omp critical(or_fft_plans)
+  {
+    for (int q = 0; q < g_nplans; q++)
+      if (g_plans[q].n == n) r = &g_plans[q];
+    if (!r && g_nplans < OR_MAX_PLANS) {
+      fft_plan *pl = &g_plans[g_nplans];
+      pl->n = n;
+      pl->nf = 0;
+      int m = n;
+      while (m > 1) {
+        int p = 2;
+        while (m % p) p++;
+        pl->f[pl->nf++] = p;
+        m /= p;
+      }
+      pl->tw = (double *)malloc(2 * (size_t)n * sizeof(double));
+      const long double twopi = 6.283185307179586476925286766559005768L;
+      for (int q = 0; q < n; q++) {
+        long double ang = twopi * (long double)q / (long double)n;
+        pl->tw[2 * q] = (double)cosl(ang);
+        pl->tw[2 * q + 1] = (double)(-sinl(ang));
+      }
+      g_nplans++;
+      r = pl;
+    }
+  }
+  return r;
 }
 
-static void fft_rec(int n, const double *x, int stride, double *out, int sign, double *work) {
+/* workspace (doubles) needed by fft_rec below for a plan of length n */
+static size_t fft_ws_size(int n) { return 6 * (size_t)n + 8; }
+
+static void fft_rec(const fft_plan *pl, int level, int n, const double *x, int stride, double *out,
+                    int sign, double *work) {
   if (n == 1) {
     out[0] = x[0];
     out[1] = x[1];
     return;
   }
-  int p = 2;
-  while (n % p) p++;
-  int m = n / p;
-  /* sub-transforms of x[r::p] into work[r*m .. r*m+m-1] */
-  double *sub = work;
-  double *wk2 = work + 2 * (size_t)n;
-  for (int r = 0; r < p; r++) fft_rec(m, x + 2 * (size_t)r * stride, stride * p, sub + 2 * (size_t)r * m, sign, wk2);
+  const int p = pl->f[level], m = n / p;
+  double *sub = work;                 /* p sub-transforms of length m */
+  double *wk2 = work + 2 * (size_t)n; /* deeper levels */
+  for (int r = 0; r < p; r++)
+    fft_rec(pl, level + 1, m, x + 2 * (size_t)r * stride, stride * p, sub + 2 * (size_t)r * m, sign, wk2);
+  const int scale = pl->n / n;
+  const double sg = (sign < 0) ? 1.0 : -1.0; /* table holds exp(-i...) */
   for (int k = 0; k < n; k++) {
-    double acc[2] = {0.0, 0.0};
+    double ar = 0.0, ai = 0.0;
+    const int km = k % m;
     for (int r = 0; r < p; r++) {
-      double w[2], t[2];
-      twiddle(r * k, n, sign, w);
-      const double *v = sub + 2 * ((size_t)r * m + (k % m));
-      t[0] = w[0] * v[0] - w[1] * v[1];
-      t[1] = w[0] * v[1] + w[1] * v[0];
-      acc[0] += t[0];
-      acc[1] += t[1];
+      const int e = (int)(((long long)r * k) % n) * scale;
+      const double wr = pl->tw[2 * e], wi = sg * pl->tw[2 * e + 1];
+      const double *v = sub + 2 * ((size_t)r * m + km);
+      ar += wr * v[0] - wi * v[1];
+      ai += wr * v[1] + wi * v[0];
     }
-    out[2 * k] = acc[0];
-    out[2 * k + 1] = acc[1];
+    out[2 * k] = ar;
+    out[2 * k + 1] = ai;
   }
 }
 
-static size_t fft_work_size(int n) {
-  size_t tot = 0;
-  while (n > 1) {
-    tot += 2 * (size_t)n;
-    int p = 2;
-    while (n % p) p++;
-    n /= p;
-  }
-  return tot + 2;
-}
-
-void or_fft_c2c(int n, double *x, int sign) {
-  double *work = (double *)malloc(fft_work_size(n) * sizeof(double));
-  double *out = (double *)malloc(2 * (size_t)n * sizeof(double));
-  fft_rec(n, x, 1, out, sign, work);
+/* in-place c2c of x (interleaved); ws: fft_ws_size(n) + 2n doubles */
+static void fft_c2c_ws(const fft_plan *pl, double *x, int sign, double *ws) {
+  const int n = pl->n;
+  double *out = ws + fft_ws_size(n);
+  fft_rec(pl, 0, n, x, 1, out, sign, ws);
   memcpy(x, out, 2 * (size_t)n * sizeof(double));
-  free(out);
-  free(work);
 }
 
-void or_fft_r2c(int n, const double *in, double *out) {
-  double *x = (double *)malloc(2 * (size_t)n * sizeof(double));
+/* r2c: n reals -> n/2+1 complex; ws: fft_ws_size(n) + 4n doubles */
+static void fft_r2c_ws(const fft_plan *pl, const double *in, double *out, double *ws) {
+  const int n = pl->n;
+  double *x = ws + fft_ws_size(n) + 2 * (size_t)n;
   for (int q = 0; q < n; q++) {
     x[2 * q] = in[q];
     x[2 * q + 1] = 0.0;
   }
-  or_fft_c2c(n, x, -1);
+  fft_c2c_ws(pl, x, -1, ws);
   memcpy(out, x, 2 * (size_t)(n / 2 + 1) * sizeof(double));
-  free(x);
 }
 
-/* Hermitian extension of the n/2+1 inputs, backward DFT, real part. */
-void or_fft_c2r(int n, const double *in, double *out) {
-  double *x = (double *)malloc(2 * (size_t)n * sizeof(double));
-  int h = n / 2 + 1;
+/* c2r: Hermitian extension of the n/2+1 inputs, backward DFT, real part */
+static void fft_c2r_ws(const fft_plan *pl, const double *in, double *out, double *ws) {
+  const int n = pl->n, h = n / 2 + 1;
+  double *x = ws + fft_ws_size(n) + 2 * (size_t)n;
   for (int q = 0; q < h; q++) {
     x[2 * q] = in[2 * q];
     x[2 * q + 1] = in[2 * q + 1];
@@ -1271,9 +1295,30 @@ void or_fft_c2r(int n, const double *in, double *out) {
     x[2 * q] = in[2 * (n - q)];
     x[2 * q + 1] = -in[2 * (n - q) + 1];
   }
-  or_fft_c2c(n, x, +1);
+  fft_c2c_ws(pl, x, +1, ws);
   for (int q = 0; q < n; q++) out[q] = x[2 * q];
-  free(x);
+}
+
+static double *fft_ws_alloc(int n) {
+  return (double *)malloc((fft_ws_size(n) + 4 * (size_t)n) * sizeof(double));
+}
+
+void or_fft_c2c(int n, double *x, int sign) {
+  double *ws = fft_ws_alloc(n);
+  fft_c2c_ws(fft_get_plan(n), x, sign, ws);
+  free(ws);
+}
+
+void or_fft_r2c(int n, const double *in, double *out) {
+  double *ws = fft_ws_alloc(n);
+  fft_r2c_ws(fft_get_plan(n), in, out, ws);
+  free(ws);
+}
+
+void or_fft_c2r(int n, const double *in, double *out) {
+  double *ws = fft_ws_alloc(n);
+  fft_c2r_ws(fft_get_plan(n), in, out, ws);
+  free(ws);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1285,17 +1330,19 @@ void or_solve_pressure_correction(or_state *s) {
   const int nzmh = nzm / 2 + 1;
   size_t ns = (size_t)nrm * nzmh * nthm;
   double *spec = (double *)malloc(2 * ns * sizeof(double));
+  const fft_plan *plz = fft_get_plan(nzm), *plt = fft_get_plan(nthm);
 #define SP(k, j, i) (spec + 2 * ((size_t)((k) - 1) + (size_t)nrm * ((size_t)((j) - 1) + (size_t)nzmh * (size_t)((i) - 1))))
   /* r2c along z for every (r, theta) (fwd_guruplan_y, :150) */
 #pragma omp parallel
   {
     double *line = (double *)malloc((size_t)nzm * sizeof(double));
     double *cl = (double *)malloc(2 * (size_t)nzmh * sizeof(double));
+    double *ws = fft_ws_alloc(nzm);
 #pragma omp for schedule(static)
     for (int i = 1; i <= nthm; i++)
       for (int k = 1; k <= nrm; k++) {
         for (int j = 1; j <= nzm; j++) line[j - 1] = DPH(k, j, i);
-        or_fft_r2c(nzm, line, cl);
+        fft_r2c_ws(plz, line, cl, ws);
         for (int j = 1; j <= nzmh; j++) {
           SP(k, j, i)[0] = cl[2 * (j - 1)];
           SP(k, j, i)[1] = cl[2 * (j - 1) + 1];
@@ -1303,6 +1350,7 @@ void or_solve_pressure_correction(or_state *s) {
       }
     free(line);
     free(cl);
+    free(ws);
   }
   /* c2c forward along theta for every (r, kz) (fwd_guruplan_z, :154), then
      divide by nthm*nzm (:157) */
@@ -1310,6 +1358,7 @@ void or_solve_pressure_correction(or_state *s) {
 #pragma omp parallel
   {
     double *cl = (double *)malloc(2 * (size_t)nthm * sizeof(double));
+    double *ws = fft_ws_alloc(nthm);
 #pragma omp for schedule(static)
     for (int j = 1; j <= nzmh; j++)
       for (int k = 1; k <= nrm; k++) {
@@ -1317,13 +1366,14 @@ void or_solve_pressure_correction(or_state *s) {
           cl[2 * (i - 1)] = SP(k, j, i)[0];
           cl[2 * (i - 1) + 1] = SP(k, j, i)[1];
         }
-        or_fft_c2c(nthm, cl, -1);
+        fft_c2c_ws(plt, cl, -1, ws);
         for (int i = 1; i <= nthm; i++) {
           SP(k, j, i)[0] = cl[2 * (i - 1)] / norm;
           SP(k, j, i)[1] = cl[2 * (i - 1) + 1] / norm;
         }
       }
     free(cl);
+    free(ws);
   }
   /* complex tridiagonal per (kz=j, ktheta=i), :171-192 */
 #pragma omp parallel
@@ -1369,6 +1419,7 @@ void or_solve_pressure_correction(or_state *s) {
 #pragma omp parallel
   {
     double *cl = (double *)malloc(2 * (size_t)nthm * sizeof(double));
+    double *ws = fft_ws_alloc(nthm);
 #pragma omp for schedule(static)
     for (int j = 1; j <= nzmh; j++)
       for (int k = 1; k <= nrm; k++) {
@@ -1376,19 +1427,21 @@ void or_solve_pressure_correction(or_state *s) {
           cl[2 * (i - 1)] = SP(k, j, i)[0];
           cl[2 * (i - 1) + 1] = SP(k, j, i)[1];
         }
-        or_fft_c2c(nthm, cl, +1);
+        fft_c2c_ws(plt, cl, +1, ws);
         for (int i = 1; i <= nthm; i++) {
           SP(k, j, i)[0] = cl[2 * (i - 1)];
           SP(k, j, i)[1] = cl[2 * (i - 1) + 1];
         }
       }
     free(cl);
+    free(ws);
   }
   /* c2r along z into dph (:204-207) */
 #pragma omp parallel
   {
     double *line = (double *)malloc((size_t)nzm * sizeof(double));
     double *cl = (double *)malloc(2 * (size_t)nzmh * sizeof(double));
+    double *ws = fft_ws_alloc(nzm);
 #pragma omp for schedule(static)
     for (int i = 1; i <= nthm; i++)
       for (int k = 1; k <= nrm; k++) {
@@ -1396,11 +1449,12 @@ void or_solve_pressure_correction(or_state *s) {
           cl[2 * (j - 1)] = SP(k, j, i)[0];
           cl[2 * (j - 1) + 1] = SP(k, j, i)[1];
         }
-        or_fft_c2r(nzm, cl, line);
+        fft_c2r_ws(plz, cl, line, ws);
         for (int j = 1; j <= nzm; j++) DPH(k, j, i) = line[j - 1];
       }
     free(line);
     free(cl);
+    free(ws);
   }
 #undef SP
   free(spec);

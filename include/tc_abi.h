@@ -162,6 +162,11 @@ int tc_kernel_timing(tc_ctx *ctx, int enable);
 int tc_kernel_timing_read(tc_ctx *ctx, const char *kernel, double *total_ms,
                           long long *count, int reset);
 
+/* Device-to-device copy of one whole field (nr planes) into another, with a
+ * plain streaming kernel; used as the reference point when converting
+ * hardware byte counters into bytes per launch. */
+int tc_copy_field(tc_ctx *ctx, int dst, int src);
+
 /* Device bytes held by the context (fields + scratch). */
 int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
 

@@ -56,7 +56,7 @@ ABI_SYMBOLS = [
     "tc_correct_velocity", "tc_correct_pressure", "tc_time_marcher", "tc_set_fused",
     "tc_calc_max_cfl", "tc_check_divergence", "tc_calc_torque", "tc_calc_velocity_stats",
     "tc_step", "tc_project", "tc_init_perturbed_couette", "tc_synchronize",
-    "tc_kernel_timing", "tc_kernel_timing_read", "tc_device_bytes",
+    "tc_kernel_timing", "tc_kernel_timing_read", "tc_copy_field", "tc_device_bytes",
 ]
 
 
@@ -119,6 +119,7 @@ def lib():
     L.tc_kernel_timing_read.argtypes = [_V, ctypes.c_char_p, _D,
                                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
     L.tc_device_bytes.argtypes = [_V, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.tc_copy_field.argtypes = [_V, ctypes.c_int, ctypes.c_int]
     _LIB = L
     return L
 
@@ -295,6 +296,9 @@ class Solver:
         _check(lib().tc_kernel_timing_read(self.h, name.encode(), ctypes.byref(ms),
                                            ctypes.byref(cnt), int(bool(reset))))
         return ms.value, cnt.value
+
+    def copy_field(self, dst, src):
+        _check(lib().tc_copy_field(self.h, FIELDS[dst], FIELDS[src]))
 
     def device_bytes(self):
         v = ctypes.c_ulonglong()

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -224,15 +225,15 @@ int read_reduction(tc_ctx *c, int nv, double *out) {
 
 int run_substep(tc_ctx *c) {
   if (c->fused) {
-    tc::launch_explicit_all(c);
+    tc::launch_helmholtz_fused(c);
   } else {
     tc::launch_explicit_qt(c);
     tc::launch_explicit_qr(c);
     tc::launch_explicit_qz(c);
+    tc::launch_implicit_qt(c);
+    tc::launch_implicit_qr(c);
+    tc::launch_implicit_qz(c);
   }
-  tc::launch_implicit_qt(c);
-  tc::launch_implicit_qr(c);
-  tc::launch_implicit_qz(c);
   tc::launch_divergence(c, 1.0 / (c->al * c->dt));
   tc::launch_poisson(c);
   if (c->fused) {
@@ -260,6 +261,22 @@ void set_substep(tc_ctx *c, int ns) {
   c->al = c->g.alm[ns];
   c->ga = c->g.gam[ns];
   c->ro = c->g.rom[ns];
+}
+
+
+// CalcTorque.F90:121-150: normalisation of the masked wall sums
+void torque_from_sums(const tc_ctx *c, double slow, double sup, double *inner, double *outer) {
+  const tc_params &p = c->p;
+  const double reninv = 1.0 / p.ren;
+  const double jlam = 2.0 * p.rint * p.rint * p.rext * p.rext * (p.virotint / p.rint - p.virotext / p.rext) /
+                      (p.rext * p.rext - p.rint * p.rint) * reninv;
+  const double jlami = 1.0 / jlam;
+  const double rc2 = c->g.rc[2], rcn = c->g.rc[c->nrm];
+  const double lo = slow * (rc2 * rc2 * rc2) * reninv * jlami;
+  const double up = sup * (rcn * rcn * rcn) * reninv * jlami;
+  const double my_surface = 1.0 / (c->nthm * c->nzm);
+  *inner = lo * my_surface;
+  *outer = up * my_surface;
 }
 
 }  // namespace
@@ -312,8 +329,15 @@ int tc_create(const tc_params *p, tc_ctx **out) {
   if (!c) return fail("out of host memory");
   c->p = *p;
   std::string e = tc::build_grid(*p, c->g);
-  if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, 64);
-  if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, 16);
+  // theta: row-pair FFTs, up to 32 KiB of LDS per block; z: column tiles,
+  // up to 64 KiB, as wide as that allows (wider tiles = longer row segments)
+  {
+    // LDS budgets (bytes) of the FFT tiles; TC_FFT_T_LDS / TC_FFT_Z_LDS override
+    const char *et = std::getenv("TC_FFT_T_LDS"), *ez = std::getenv("TC_FFT_Z_LDS");
+    const int lt = et ? std::atoi(et) : 16384, lz = ez ? std::atoi(ez) : 32768;
+    if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, lt);
+    if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, lz);
+  }
   if (!e.empty()) {
     delete c;
     return fail(e);
@@ -423,10 +447,16 @@ int tc_create(const tc_params *p, tc_ctx **out) {
     HTRY(hipMemcpy(c->tw_z, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
   }
   // reductions
-  TRY(dev_alloc(c, (void **)&c->red, sizeof(double) * 1024 * 8));
+  // per-block partials of the reductions: one block of 256 per (j, i) column group
+  TRY(dev_alloc(c, (void **)&c->red, sizeof(double) * 16 * (size_t)((c->P + 255) / 256)));
   TRY(dev_alloc(c, (void **)&c->red_out, sizeof(double) * 16));
   HTRY(hipHostMalloc((void **)&c->red_host, sizeof(double) * 16, hipHostMallocDefault));
   set_substep(c, 1);
+  {
+    // lines per thread of the fused Helmholtz kernel (TC_HELM_LPT overrides)
+    const char *lp = std::getenv("TC_HELM_LPT");
+    c->helm_lpt = lp ? std::max(1, std::min(2, std::atoi(lp))) : tc::choose_helm_lpt(c);
+  }
   TRY(refresh_tables(c));
   HTRY(hipStreamSynchronize(c->stream));
 #undef TRY
@@ -449,6 +479,7 @@ int tc_upload_field(tc_ctx *c, int field, const double *host, int layout) {
   CTXCHK(c);
   if (check_field(c, field)) return -1;
   if (!host) return fail("null host pointer");
+  c->cfl_valid = false;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (layout == TC_LAYOUT_DEVICE) {
     HIPCHK(hipMemcpy(c->f[field], host, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice));
@@ -541,6 +572,7 @@ int tc_set_fused(tc_ctx *c, int fused) {
 #define ROUTINE(name, body) \
   int name(tc_ctx *c) {     \
     CTXCHK(c);              \
+    c->cfl_valid = false;   \
     body;                   \
     return launch_status(); \
   }
@@ -558,6 +590,7 @@ ROUTINE(tc_correct_pressure, tc::launch_correct_pressure(c))
 
 int tc_time_marcher(tc_ctx *c) {
   CTXCHK(c);
+  c->cfl_valid = false;
   int rc = begin_step(c);
   if (rc) return rc;
   for (int ns = 1; ns <= c->p.nsst; ns++) {
@@ -570,6 +603,7 @@ int tc_time_marcher(tc_ctx *c) {
 
 int tc_project(tc_ctx *c) {
   CTXCHK(c);
+  c->cfl_valid = false;
   const double al = c->al, dt = c->dt;
   c->al = 1.0;
   c->dt = 1.0;
@@ -612,18 +646,7 @@ int tc_calc_torque(tc_ctx *c, double *inner, double *outer) {
   double s[2];
   rc = read_reduction(c, 2, s);
   if (rc) return rc;
-  // CalcTorque.F90:121-150
-  const tc_params &p = c->p;
-  const double reninv = 1.0 / p.ren;
-  const double jlam = 2.0 * p.rint * p.rint * p.rext * p.rext * (p.virotint / p.rint - p.virotext / p.rext) /
-                      (p.rext * p.rext - p.rint * p.rint) * reninv;
-  const double jlami = 1.0 / jlam;
-  const double rc2 = c->g.rc[2], rcn = c->g.rc[c->nrm];
-  const double lo = s[0] * (rc2 * rc2 * rc2) * reninv * jlami;
-  const double up = s[1] * (rcn * rcn * rcn) * reninv * jlami;
-  const double my_surface = 1.0 / (c->nthm * c->nzm);
-  *inner = lo * my_surface;
-  *outer = up * my_surface;
+  torque_from_sums(c, s[0], s[1], inner, outer);
   return 0;
 }
 
@@ -649,8 +672,13 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
   // main.F90:156-223
   int errorcode = 0;
   double instcfl;
-  int rc = tc_calc_max_cfl(c, &instcfl);
-  if (rc) return rc;
+  int rc = 0;
+  if (c->cfl_valid) {
+    instcfl = c->cfl_cached;  // same fields as the previous logging pass
+  } else {
+    rc = tc_calc_max_cfl(c, &instcfl);
+    if (rc) return rc;
+  }
   const tc_params &p = c->p;
   if (p.idtv != 0) {
     if (ntime > 1) {
@@ -671,19 +699,22 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
   c->time = c->time + c->dt;
   out[2] = c->time;
   if (ntime == 1 || std::fmod(c->time, p.tout) < c->dt) {
-    double vmax[3], rms[3];
-    rc = tc_calc_velocity_stats(c, vmax, rms);
+    // CalcVelocityStats, CalcMaxCFL, CheckDivergence, CalcTorque in one pass
+    tc::launch_diagnostics(c);
+    rc = launch_status();
     if (rc) return rc;
-    if (vmax[0] > p.limitvel && vmax[1] > p.limitvel) errorcode = 266;
-    rc = tc_calc_max_cfl(c, &out[0]);
+    double v[10];
+    rc = read_reduction(c, 10, v);
     if (rc) return rc;
-    rc = tc_check_divergence(c, &out[3]);
-    if (rc) return rc;
-    rc = tc_calc_torque(c, &out[4], &out[5]);
-    if (rc) return rc;
-    out[6] = vmax[0];
-    out[7] = vmax[1];
-    out[8] = vmax[2];
+    if (v[2] > p.limitvel && v[3] > p.limitvel) errorcode = 266;
+    out[0] = std::max(0.00000001, v[0]);
+    out[3] = v[1];
+    torque_from_sums(c, v[8], v[9], &out[4], &out[5]);
+    out[6] = v[2];
+    out[7] = v[3];
+    out[8] = v[4];
+    c->cfl_cached = out[0];
+    c->cfl_valid = true;
     if (std::fabs(out[3]) > p.resid) errorcode = 169;
   }
   if (c->time > p.tmax) errorcode = 333;
@@ -763,6 +794,15 @@ int tc_kernel_timing_read(tc_ctx *c, const char *kernel, double *total_ms, long 
   if (count) *count = (it == c->timing_acc.end()) ? 0 : it->second.second;
   if (reset) c->timing_acc.clear();
   return 0;
+}
+
+int tc_copy_field(tc_ctx *c, int dst, int src) {
+  CTXCHK(c);
+  if (check_field(c, dst) || check_field(c, src)) return -1;
+  if (dst == src) return fail("tc_copy_field needs two different fields");
+  c->cfl_valid = false;
+  tc::launch_copy(c, dst, src);
+  return launch_status();
 }
 
 int tc_device_bytes(tc_ctx *c, unsigned long long *bytes) {

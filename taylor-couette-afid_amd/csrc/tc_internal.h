@@ -31,6 +31,7 @@ struct FFTPlan {
   int nst = 0;
   int radix[24] = {0};
   int lines = 1;  // FFT lines per 256-thread block
+  int me = 8;     // points per thread per pass (n <= me * 256/lines)
 };
 
 // Helmholtz LU tables: per (substep slot, component, mask variant) a block of
@@ -78,6 +79,10 @@ struct tc_ctx {
   double time = 0, dt = 0, beta = 0, al = 0, ga = 0, ro = 0, vinner_t = 0, dvinner_t = 0;
   int ns = 1;
   bool fused = true;
+  int helm_lpt = 1;  // radial lines per thread in the fused Helmholtz kernel (1 or 2)
+  // CFL of the current velocity, when known (set by the logging pass of tc_step)
+  bool cfl_valid = false;
+  double cfl_cached = 0.0;
   // timing
   bool timing = false;
   std::vector<tc::Timer> timers;
@@ -92,7 +97,8 @@ namespace tc {
 void launch_explicit_qt(tc_ctx *c);
 void launch_explicit_qr(tc_ctx *c);
 void launch_explicit_qz(tc_ctx *c);
-void launch_explicit_all(tc_ctx *c);
+void launch_helmholtz_fused(tc_ctx *c);
+int choose_helm_lpt(const tc_ctx *c);
 void launch_implicit_qt(tc_ctx *c);
 void launch_implicit_qr(tc_ctx *c);
 void launch_implicit_qz(tc_ctx *c);
@@ -106,9 +112,12 @@ void launch_max_cfl(tc_ctx *c);
 void launch_max_div(tc_ctx *c);
 void launch_torque(tc_ctx *c);
 void launch_velocity_stats(tc_ctx *c);
+// one pass: cfl, max|div|, vmax(3), rms sums(3), torque sums(2)
+void launch_diagnostics(tc_ctx *c);
+void launch_copy(tc_ctx *c, int dst, int src);
 
 // Poisson solve (tc_poisson.hip)
-std::string make_fft_plan(int n, FFTPlan &pl, int max_lines);
+std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes);
 void fill_twiddles(int n, std::vector<double> &tw);
 void launch_poisson(tc_ctx *c);
 

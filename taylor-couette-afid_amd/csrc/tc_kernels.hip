@@ -18,28 +18,53 @@ DEVI int wrap_p(int i, int n) { return (i + 1 == n) ? 0 : i + 1; }
 DEVI int wrap_m(int i, int n) { return (i == 0) ? n - 1 : i - 1; }
 
 // ---------------------------------------------------------------------------
-// Explicit terms, one point per thread.
+// Explicit terms.  Written once, generic over an accessor A with
+// A(dk, dj, di) = field value at (k+dk, j+dj, i+di) of the point being
+// computed, so the routine-level kernels (global loads) and the fused
+// Helmholtz kernel (LDS tiles) evaluate identical arithmetic.
 // ---------------------------------------------------------------------------
 
+// Global-memory accessor with periodic wrap in z and theta.
+struct GAcc {
+  const double *__restrict__ a;
+  long long base;  // IX(k, j, i)
+  long long P;
+  int i, j, nthm, nzm;
+  DEVI double operator()(int dk, int dj, int di) const {
+    int jj = j + dj, ii = i + di;
+    jj = (jj < 0) ? jj + nzm : (jj >= nzm ? jj - nzm : jj);
+    ii = (ii < 0) ? ii + nthm : (ii >= nthm ? ii - nthm : ii);
+    return a[base + dk * P + (long long)(jj - j) * nthm + (ii - i)];
+  }
+};
+DEVI GAcc gacc(const GridDev &g, const double *a, int k, int j, int i) {
+  GAcc r;
+  r.a = a;
+  r.base = IX(g, k, j, i);
+  r.P = g.P;
+  r.i = i;
+  r.j = j;
+  r.nthm = g.nthm;
+  r.nzm = g.nzm;
+  return r;
+}
+
 // ExplicitTermsQT.F90:107-192 at (kc, j, i)
-DEVI double explicit_qt_at(const GridDev &g, const double *__restrict__ qt,
-                           const double *__restrict__ qr, const double *__restrict__ qz, int kc,
-                           int j, int i) {
-  const int ip = wrap_p(i, g.nthm), im = wrap_m(i, g.nthm);
-  const int jp = wrap_p(j, g.nzm), jm = wrap_m(j, g.nzm);
-  const int kmm = kc > 1 ? kc - 1 : 1, kpp = kc < g.nrm ? kc + 1 : g.nrm, kp = kc + 1;
+template <class A>
+DEVI double explicit_qt_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
+  const int dkm = kc > 1 ? -1 : 0, dkp = kc < g.nrm ? 1 : 0;  // kmv, kpv clamps
   const double udxt = g.dxt * 0.25 * g.usrm[kc];
   const double udxtq = g.dxtq / (g.ren * (g.rm[kc] * g.rm[kc]));
   const double udxzq = g.dxzq / g.ren;
   const double udxz = g.dxz * 0.25;
-  const double t_c = qt[IX(g, kc, j, i)], t_ip = qt[IX(g, kc, j, ip)], t_im = qt[IX(g, kc, j, im)];
-  const double t_jp = qt[IX(g, kc, jp, i)], t_jm = qt[IX(g, kc, jm, i)];
-  const double t_kp = qt[IX(g, kpp, j, i)], t_km = qt[IX(g, kmm, j, i)];
-  const double r_c = qr[IX(g, kc, j, i)], r_im = qr[IX(g, kc, j, im)];
-  const double r_kp = qr[IX(g, kp, j, i)], r_kpim = qr[IX(g, kp, j, im)];
-  const double z_c = qz[IX(g, kc, j, i)], z_im = qz[IX(g, kc, j, im)];
-  const double z_jp = qz[IX(g, kc, jp, i)], z_jpim = qz[IX(g, kc, jp, im)];
-  const double usrc_kp = g.usrc[kp], usrc_k = g.usrc[kc];
+  const double t_c = T(0, 0, 0), t_ip = T(0, 0, 1), t_im = T(0, 0, -1);
+  const double t_jp = T(0, 1, 0), t_jm = T(0, -1, 0);
+  const double t_kp = T(dkp, 0, 0), t_km = T(dkm, 0, 0);
+  const double r_c = R(0, 0, 0), r_im = R(0, 0, -1);
+  const double r_kp = R(1, 0, 0), r_kpim = R(1, 0, -1);
+  const double z_c = Z(0, 0, 0), z_im = Z(0, 0, -1);
+  const double z_jp = Z(0, 1, 0), z_jpim = Z(0, 1, -1);
+  const double usrc_kp = g.usrc[kc + 1], usrc_k = g.usrc[kc];
 
   const double h11 = ((t_ip + t_c) * (t_ip + t_c) - (t_c + t_im) * (t_c + t_im)) * udxt;
   const double h12d = ((r_kp + r_kpim) * (t_kp + t_c) - (r_c + r_im) * (t_c + t_km)) * 0.25 * g.udx2rm[kc];
@@ -56,29 +81,25 @@ DEVI double explicit_qt_at(const GridDev &g, const double *__restrict__ qt,
 }
 
 // ExplicitTermsQR.F90:247-347 at (kc, j, i), kc = 2..nrm
-DEVI double explicit_qr_at(const GridDev &g, const double *__restrict__ qt,
-                           const double *__restrict__ qr, const double *__restrict__ qz, int kc,
-                           int j, int i) {
-  const int ip = wrap_p(i, g.nthm), im = wrap_m(i, g.nthm);
-  const int jp = wrap_p(j, g.nzm), jm = wrap_m(j, g.nzm);
-  const int kmm = kc - 1, kp = kc + 1;
+template <class A>
+DEVI double explicit_qr_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
   const double udxt = g.dxt * 0.25 * g.usrc[kc];
   const double udxtq = g.dxtq / (g.ren * (g.rc[kc] * g.rc[kc]));
   const double udxzq = g.dxzq / g.ren;
   const double udxz = g.dxz * 0.25;
-  const double t_c = qt[IX(g, kc, j, i)], t_ip = qt[IX(g, kc, j, ip)];
-  const double t_m = qt[IX(g, kmm, j, i)], t_mip = qt[IX(g, kmm, j, ip)];
-  const double r_c = qr[IX(g, kc, j, i)], r_ip = qr[IX(g, kc, j, ip)], r_im = qr[IX(g, kc, j, im)];
-  const double r_jp = qr[IX(g, kc, jp, i)], r_jm = qr[IX(g, kc, jm, i)];
-  const double r_kp = qr[IX(g, kp, j, i)], r_km = qr[IX(g, kmm, j, i)];
-  const double z_c = qz[IX(g, kc, j, i)], z_jp = qz[IX(g, kc, jp, i)];
-  const double z_m = qz[IX(g, kmm, j, i)], z_mjp = qz[IX(g, kmm, jp, i)];
+  const double t_c = T(0, 0, 0), t_ip = T(0, 0, 1);
+  const double t_m = T(-1, 0, 0), t_mip = T(-1, 0, 1);
+  const double r_c = R(0, 0, 0), r_ip = R(0, 0, 1), r_im = R(0, 0, -1);
+  const double r_jp = R(0, 1, 0), r_jm = R(0, -1, 0);
+  const double r_kp = R(1, 0, 0), r_km = R(-1, 0, 0);
+  const double z_c = Z(0, 0, 0), z_jp = Z(0, 1, 0);
+  const double z_m = Z(-1, 0, 0), z_mjp = Z(-1, 1, 0);
 
   const double tavg = (t_ip + t_mip + t_c + t_m) * 0.25;
   const double h11n = tavg * tavg;
   const double h21 = ((t_ip + t_mip) * (r_ip + r_c) - (t_c + t_m) * (r_c + r_im)) * udxt;
-  const double h22 = ((r_kp * g.usrc[kp] + r_c * g.usrc[kc]) * (r_kp + r_c) -
-                      (r_c * g.usrc[kc] + r_km * g.usrc[kmm]) * (r_c + r_km)) *
+  const double h22 = ((r_kp * g.usrc[kc + 1] + r_c * g.usrc[kc]) * (r_kp + r_c) -
+                      (r_c * g.usrc[kc] + r_km * g.usrc[kc - 1]) * (r_c + r_km)) *
                      g.udx2c[kc] * 0.25;
   const double h23 = ((z_jp + z_mjp) * (r_jp + r_c) - (z_c + z_m) * (r_c + r_jm)) * udxz;
   const double qte = (t_ip + t_mip);
@@ -91,23 +112,20 @@ DEVI double explicit_qr_at(const GridDev &g, const double *__restrict__ qt,
 }
 
 // ExplicitTermsQZ.F90:395-457 at (kc, j, i)
-DEVI double explicit_qz_at(const GridDev &g, const double *__restrict__ qt,
-                           const double *__restrict__ qr, const double *__restrict__ qz, int kc,
-                           int j, int i) {
-  const int ip = wrap_p(i, g.nthm), im = wrap_m(i, g.nthm);
-  const int jp = wrap_p(j, g.nzm), jm = wrap_m(j, g.nzm);
-  const int kmm = kc > 1 ? kc - 1 : 1, kpp = kc < g.nrm ? kc + 1 : g.nrm, kp = kc + 1;
+template <class A>
+DEVI double explicit_qz_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
+  const int dkm = kc > 1 ? -1 : 0, dkp = kc < g.nrm ? 1 : 0;
   const double udxt = g.dxt * 0.25 * g.usrm[kc];
   const double udxtq = g.dxtq / (g.ren * (g.rm[kc] * g.rm[kc]));
   const double udxzq = g.dxzq / g.ren;
   const double udxz = g.dxz * 0.25;
-  const double t_c = qt[IX(g, kc, j, i)], t_ip = qt[IX(g, kc, j, ip)];
-  const double t_jm = qt[IX(g, kc, jm, i)], t_jmip = qt[IX(g, kc, jm, ip)];
-  const double r_c = qr[IX(g, kc, j, i)], r_jm = qr[IX(g, kc, jm, i)];
-  const double r_kp = qr[IX(g, kp, j, i)], r_kpjm = qr[IX(g, kp, jm, i)];
-  const double z_c = qz[IX(g, kc, j, i)], z_ip = qz[IX(g, kc, j, ip)], z_im = qz[IX(g, kc, j, im)];
-  const double z_jp = qz[IX(g, kc, jp, i)], z_jm = qz[IX(g, kc, jm, i)];
-  const double z_kp = qz[IX(g, kpp, j, i)], z_km = qz[IX(g, kmm, j, i)];
+  const double t_c = T(0, 0, 0), t_ip = T(0, 0, 1);
+  const double t_jm = T(0, -1, 0), t_jmip = T(0, -1, 1);
+  const double r_c = R(0, 0, 0), r_jm = R(0, -1, 0);
+  const double r_kp = R(1, 0, 0), r_kpjm = R(1, -1, 0);
+  const double z_c = Z(0, 0, 0), z_ip = Z(0, 0, 1), z_im = Z(0, 0, -1);
+  const double z_jp = Z(0, 1, 0), z_jm = Z(0, -1, 0);
+  const double z_kp = Z(dkp, 0, 0), z_km = Z(dkm, 0, 0);
 
   const double h31 = ((t_ip + t_jmip) * (z_ip + z_c) - (t_c + t_jm) * (z_c + z_im)) * udxt;
   const double h32 = (((r_kp + r_kpjm) * (z_kp + z_c)) - ((r_c + r_jm) * (z_c + z_km))) * g.udx2rm[kc] * 0.25;
@@ -125,33 +143,16 @@ __global__ __launch_bounds__(256) void k_explicit(GridDev g, const double *__res
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y * 4 + threadIdx.y;
   if (i >= g.nthm || j >= g.nzm) return;
-  if (WHICH == 0) {
-    const int kc = blockIdx.z + 1;
-    out[IX(g, kc, j, i)] = explicit_qt_at(g, qt, qr, qz, kc, j, i);
-  } else if (WHICH == 1) {
-    const int kc = blockIdx.z + 2;
-    out[IX(g, kc, j, i)] = explicit_qr_at(g, qt, qr, qz, kc, j, i);
-  } else {
-    const int kc = blockIdx.z + 1;
-    out[IX(g, kc, j, i)] = explicit_qz_at(g, qt, qr, qz, kc, j, i);
-  }
-}
-
-// All three explicit terms in one pass over the velocity (same arithmetic).
-__global__ __launch_bounds__(256) void k_explicit_all(GridDev g, const double *__restrict__ qt,
-                                                      const double *__restrict__ qr,
-                                                      const double *__restrict__ qz,
-                                                      double *__restrict__ dq,
-                                                      double *__restrict__ dph,
-                                                      double *__restrict__ qcap) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= g.nthm || j >= g.nzm) return;
-  const int kc = blockIdx.z + 1;
-  const long long o = IX(g, kc, j, i);
-  dq[o] = explicit_qt_at(g, qt, qr, qz, kc, j, i);
-  if (kc >= 2) dph[o] = explicit_qr_at(g, qt, qr, qz, kc, j, i);
-  qcap[o] = explicit_qz_at(g, qt, qr, qz, kc, j, i);
+  const int kc = blockIdx.z + (WHICH == 1 ? 2 : 1);
+  const GAcc T = gacc(g, qt, kc, j, i), R = gacc(g, qr, kc, j, i), Z = gacc(g, qz, kc, j, i);
+  double v;
+  if (WHICH == 0)
+    v = explicit_qt_g(g, T, R, Z, kc);
+  else if (WHICH == 1)
+    v = explicit_qr_g(g, T, R, Z, kc);
+  else
+    v = explicit_qz_g(g, T, R, Z, kc);
+  out[IX(g, kc, j, i)] = v;
 }
 
 static dim3 point_grid(const tc_ctx *c, int planes) {
@@ -176,23 +177,15 @@ void launch_explicit_qz(tc_ctx *c) {
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_QCAP]);
   timer_end(c);
 }
-void launch_explicit_all(tc_ctx *c) {
-  timer_begin(c, "explicit_all");
-  hipLaunchKernelGGL(k_explicit_all, point_grid(c, c->nrm), dim3(64, 4), 0, c->stream, c->gd,
-                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_DQ], c->f[TC_DPH],
-                     c->f[TC_QCAP]);
-  timer_end(c);
-}
 
 // ---------------------------------------------------------------------------
-// Helmholtz: right-hand side assembly fused with the batched tridiagonal
-// solve.  One radial line (j, i) per lane; lanes run along theta, so every
-// load and store of a plane is coalesced.  The LU factors (dgttrf with partial
-// pivoting, row-normalised as in SolveImpEqnUpdate_*) do not depend on (j, i)
-// except through the wall masks, so they come from a per-substep table with
-// one entry per mask combination.  The forward substitution is streamed while
-// the right-hand side is produced; its output y goes to the rhs field and is
-// read back by the backward substitution.
+// Helmholtz solves.  The LU factors (dgttrf with partial pivoting of the
+// row-normalised matrices of SolveImpEqnUpdate_*) do not depend on the line
+// (j, i) except through the wall masks, so they come from a per-substep
+// table with one entry per mask combination.  One radial line per lane;
+// lanes run along theta so every plane access is coalesced.  The forward
+// substitution is streamed while the right-hand side is produced; its output
+// y is written to a scratch field and read back by the backward substitution.
 // ---------------------------------------------------------------------------
 
 struct LUView {
@@ -229,6 +222,60 @@ struct ImpArgs {
   double udxz;      // dxz*al (ImplicitAndUpdateQZ.F90:25)
 };
 
+// Right-hand side row k of the q_theta solve (ImplicitAndUpdateQT.F90:45-115);
+// qm, q0, qp = qt(k-1), qt(k), qt(k+1) on the line.
+DEVI double rhs_qt(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
+                   double pr_c, double pr_im, double dqv, double ruold, double fic, double foc) {
+  double d33;
+  if (k == 1) {
+    const double app = g.ap1j[1];
+    const double acc = (g.ac1j[1] * fic + g.ac1j_fs1 * (1 - fic));
+    const double amm = (g.am1j[1] * fic + g.am1j_fs1 * (1 - fic));
+    d33 = qp * app + q0 * acc + a.vwall_in * amm;
+  } else if (k < g.nrm) {
+    d33 = qp * g.ap1j[k] + q0 * g.ac1j[k] + qm * g.am1j[k];
+  } else {
+    const double app = (g.ap1j[k] * foc + g.ap1j_fs2 * (1 - foc));
+    const double acc = (g.ac1j[k] * foc + g.ac1j_fs2 * (1 - foc));
+    d33 = a.virotext * app + q0 * acc + qm * g.am1j[k];
+  }
+  const double dpx = (pr_c - pr_im) * g.dxt * a.al * g.usrm[k];
+  return (a.ga * dqv + a.ro * ruold + a.alre * d33 - dpx) * a.dt;
+}
+
+// Right-hand side row k = 2..nrm of the q_r solve (ImplicitAndUpdateQR.F90:249-271).
+DEVI double rhs_qr(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
+                   double pr_c, double pr_km, double dphv, double ruold) {
+  const double d33 = qp * g.ap2je[k] + q0 * g.ac2je[k] + qm * g.am2je[k];
+  const double udx2 = g.rc[k] * a.al * g.udx2c[k];
+  const double dpx = (pr_c - pr_km) * udx2;
+  return (a.ga * dphv + a.ro * ruold + a.alre * d33 - dpx) * a.dt;
+}
+
+// Right-hand side row k of the q_z solve (ImplicitAndUpdateQZ.F90:42-113).
+// Row nrm is the one the reference leaves behind: assembled a second time
+// after the loop overwrote ru3(nrm) with qcap(nrm), so its history term is
+// ro*qcap(nrm), with the loop's amm = am3j(nrm).
+DEVI double rhs_qz(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
+                   double pr_c, double pr_jm, double qcv, double ruold, double fic, double foc) {
+  double d33, hist = ruold;
+  if (k == 1) {
+    const double app = g.ap3j[1];
+    const double acc = (g.ac3j[1] * fic + g.ac3j_fs1 * (1 - fic));
+    const double amm = (g.am3j[1] * fic + g.am3j_fs1 * (1 - fic));
+    d33 = qp * app + q0 * acc + q0 * amm;
+  } else if (k < g.nrm) {
+    d33 = qp * g.ap3j[k] + q0 * g.ac3j[k] + qm * g.am3j[k];
+  } else {
+    const double app = (g.ap3j[k] * foc + g.ap3j_fs2 * (1 - foc));
+    const double acc = (g.ac3j[k] * foc + g.ac3j_fs2 * (1 - foc));
+    d33 = q0 * app + q0 * acc + qm * g.am3j[k];
+    hist = qcv;
+  }
+  const double dpx = (pr_c - pr_jm) * a.udxz;
+  return (a.ga * qcv + a.ro * hist + a.alre * d33 - dpx) * a.dt;
+}
+
 // ImplicitAndUpdateQT.F90:40-117 + SolveImpEqnUpdate_QT.F90:158-204
 __global__ __launch_bounds__(256) void k_implicit_qt(GridDev g, ImpArgs a, double *__restrict__ qt,
                                                      const double *__restrict__ pr,
@@ -241,65 +288,38 @@ __global__ __launch_bounds__(256) void k_implicit_qt(GridDev g, ImpArgs a, doubl
   const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= g.P) return;
   const int j = (int)(l / g.nthm), i = (int)(l - (long long)j * g.nthm);
-  const int im = wrap_m(i, g.nthm);
+  const long long lim = (long long)j * g.nthm + wrap_m(i, g.nthm);
   const int n = g.nrm;
   const int ic = icyl[l], oc = ocyl[l];
   const double fic = (double)ic, foc = (double)oc;
   const LUView T = lu_view(lu + (size_t)(ic + 2 * oc) * nt, nt);
   const long long P = g.P;
-  const double dxtal = g.dxt * a.al;
-
-  // k = 1 (inner cylinder rows, ImplicitAndUpdateQT.F90:45-68)
-  double qm, q0 = qt[l], qp = qt[l + P];
-  double carry, yprev;
-  {
-    const double app = g.ap1j[1];
-    const double acc = (g.ac1j[1] * fic + g.ac1j_fs1 * (1 - fic));
-    const double amm = (g.am1j[1] * fic + g.am1j_fs1 * (1 - fic));
-    const double d33 = qp * app + q0 * acc + a.vwall_in * amm;
-    const double dpx = (pr[l] - pr[l - i + im]) * dxtal * g.usrm[1];
-    const double dqv = dq[l];
-    const double r = (a.ga * dqv + a.ro * ru1[l] + a.alre * d33 - dpx) * a.dt;
-    ru1[l] = dqv;
-    carry = (r + a.bc1) * T.sc[0];
-  }
-  for (int k = 2; k <= n; k++) {
+  double qm = 0.0, q0 = qt[l], qp = qt[l + P];
+  double carry = 0.0, yprev;
+  for (int k = 1; k <= n; k++) {
     const long long o = (long long)(k - 1) * P + l;
-    qm = q0;
-    q0 = qp;
-    double d33;
-    if (k < n) {
-      qp = qt[o + P];
-      d33 = qp * g.ap1j[k] + q0 * g.ac1j[k] + qm * g.am1j[k];
-    } else {
-      const double app = (g.ap1j[k] * foc + g.ap1j_fs2 * (1 - foc));
-      const double acc = (g.ac1j[k] * foc + g.ac1j_fs2 * (1 - foc));
-      d33 = a.virotext * app + q0 * acc + qm * g.am1j[k];
+    if (k > 1) {
+      qm = q0;
+      q0 = qp;
+      qp = (k < n) ? qt[o + P] : 0.0;
     }
-    const double dpx = (pr[o] - pr[o - i + im]) * dxtal * g.usrm[k];
     const double dqv = dq[o];
-    const double r = (a.ga * dqv + a.ro * ru1[o] + a.alre * d33 - dpx) * a.dt;
+    const double r = rhs_qt(g, a, k, qm, q0, qp, pr[o], pr[o - l + lim], dqv, ru1[o], fic, foc);
     ru1[o] = dqv;
-    lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
-    y[o - P] = yprev;
+    if (k == 1) {
+      carry = (r + a.bc1) * T.sc[0];
+    } else {
+      lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
+      y[o - P] = yprev;
+    }
   }
-  // backward substitution (dgtts2), qt += x
-  double x1 = carry * T.rd[n - 1];
-  qt[(long long)(n - 1) * P + l] += x1;
-  double x0 = (y[(long long)(n - 2) * P + l] - T.du[n - 2] * x1) * T.rd[n - 2];
-  qt[(long long)(n - 2) * P + l] += x0;
-  for (int r = n - 3; r >= 0; r--) {
-    const long long o = (long long)r * P + l;
-    const double x = (y[o] - T.du[r] * x0 - T.du2[r] * x1) * T.rd[r];
-    qt[o] += x;
-    x1 = x0;
-    x0 = x;
-  }
+  y[(long long)(n - 1) * P + l] = carry;
 }
 
-// ImplicitAndUpdateQR.F90:247-279 + SolveImpEqnUpdate_QR.F90:335-349: the
-// system has nr rows with identity rows at both walls (zero right-hand side).
-__global__ __launch_bounds__(256) void k_implicit_qr(GridDev g, ImpArgs a, double *__restrict__ qr,
+// ImplicitAndUpdateQR.F90:247-271 + the forward half of SolveImpEqnUpdate_QR:
+// nr rows with identity rows (zero right-hand side) at both walls.
+__global__ __launch_bounds__(256) void k_implicit_qr(GridDev g, ImpArgs a,
+                                                     const double *__restrict__ qr,
                                                      const double *__restrict__ pr,
                                                      const double *__restrict__ dph,
                                                      double *__restrict__ ru2,
@@ -317,36 +337,20 @@ __global__ __launch_bounds__(256) void k_implicit_qr(GridDev g, ImpArgs a, doubl
     qm = q0;
     q0 = qp;
     qp = qr[o + P];
-    const double d33 = qp * g.ap2je[k] + q0 * g.ac2je[k] + qm * g.am2je[k];
-    const double udx2 = g.rc[k] * a.al * g.udx2c[k];
-    const double dpx = (pr[o] - pr[o - P]) * udx2;
     const double dv = dph[o];
-    const double r = (a.ga * dv + a.ro * ru2[o] + a.alre * d33 - dpx) * a.dt;
+    const double r = rhs_qr(g, a, k, qm, q0, qp, pr[o], pr[o - P], dv, ru2[o]);
     ru2[o] = dv;
     lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
     y[o - P] = yprev;
   }
   lsolve_step(T, n - 2, 0.0, carry, yprev);  // row nr: zero right-hand side
   y[(long long)(n - 2) * P + l] = yprev;
-  double x1 = carry * T.rd[n - 1];
-  double x0 = (yprev - T.du[n - 2] * x1) * T.rd[n - 2];
-  if (n - 2 >= 1) qr[(long long)(n - 2) * P + l] += x0;
-  for (int r = n - 3; r >= 1; r--) {
-    const long long o = (long long)r * P + l;
-    const double x = (y[o] - T.du[r] * x0 - T.du2[r] * x1) * T.rd[r];
-    qr[o] += x;
-    x1 = x0;
-    x0 = x;
-  }
-  // ImplicitAndUpdateQR.F90:278-279
-  qr[l] = 0.0;
-  qr[(long long)(n - 1) * P + l] = 0.0;
+  y[(long long)(n - 1) * P + l] = carry;
 }
 
-// ImplicitAndUpdateQZ.F90:36-115 + SolveImpEqnUpdate_QZ.F90:155-192.  Row nrm
-// is assembled the way the reference leaves it: its history term uses
-// ru3(nrm) after the loop has already overwritten it with qcap(nrm).
-__global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a, double *__restrict__ qz,
+// ImplicitAndUpdateQZ.F90:36-115 + SolveImpEqnUpdate_QZ.F90:155-186.
+__global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a,
+                                                     const double *__restrict__ qz,
                                                      const double *__restrict__ pr,
                                                      const double *__restrict__ qcap,
                                                      double *__restrict__ ru3,
@@ -357,59 +361,299 @@ __global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a, doubl
   const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= g.P) return;
   const int j = (int)(l / g.nthm), i = (int)(l - (long long)j * g.nthm);
-  const int jm = wrap_m(j, g.nzm);
-  const long long ljm = (long long)jm * g.nthm + i;
+  const long long ljm = (long long)wrap_m(j, g.nzm) * g.nthm + i;
   const int n = g.nrm;
   const int ic = icyl[l], oc = ocyl[l];
   const double fic = (double)ic, foc = (double)oc;
   const LUView T = lu_view(lu + (size_t)(ic + 2 * oc) * nt, nt);
   const long long P = g.P;
-
-  double qm, q0 = qz[l], qp = qz[l + P];
-  double carry, yprev;
-  {
-    const double app = g.ap3j[1];
-    const double acc = (g.ac3j[1] * fic + g.ac3j_fs1 * (1 - fic));
-    const double amm = (g.am3j[1] * fic + g.am3j_fs1 * (1 - fic));
-    const double d33 = qp * app + q0 * acc + q0 * amm;
-    const double dpx = (pr[l] - pr[ljm]) * a.udxz;
-    const double qv = qcap[l];
-    const double r = (a.ga * qv + a.ro * ru3[l] + a.alre * d33 - dpx) * a.dt;
-    ru3[l] = qv;
-    carry = r * T.sc[0];
-  }
-  for (int k = 2; k <= n; k++) {
+  double qm = 0.0, q0 = qz[l], qp = qz[l + P];
+  double carry = 0.0, yprev;
+  for (int k = 1; k <= n; k++) {
     const long long o = (long long)(k - 1) * P + l;
-    qm = q0;
-    q0 = qp;
-    const double qv = qcap[o];
-    double d33, hist;
-    if (k < n) {
-      qp = qz[o + P];
-      d33 = qp * g.ap3j[k] + q0 * g.ac3j[k] + qm * g.am3j[k];
-      hist = ru3[o];
-    } else {
-      const double app = (g.ap3j[k] * foc + g.ap3j_fs2 * (1 - foc));
-      const double acc = (g.ac3j[k] * foc + g.ac3j_fs2 * (1 - foc));
-      d33 = q0 * app + q0 * acc + qm * g.am3j[k];
-      hist = qv;
+    if (k > 1) {
+      qm = q0;
+      q0 = qp;
+      qp = (k < n) ? qz[o + P] : 0.0;
     }
-    const double dpx = (pr[o] - pr[(long long)(k - 1) * P + ljm]) * a.udxz;
-    const double r = (a.ga * qv + a.ro * hist + a.alre * d33 - dpx) * a.dt;
+    const double qv = qcap[o];
+    const double r = rhs_qz(g, a, k, qm, q0, qp, pr[o], pr[o - l + ljm], qv, ru3[o], fic, foc);
     ru3[o] = qv;
-    lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
-    y[o - P] = yprev;
+    if (k == 1) {
+      carry = r * T.sc[0];
+    } else {
+      lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
+      y[o - P] = yprev;
+    }
   }
-  double x1 = carry * T.rd[n - 1];
-  qz[(long long)(n - 1) * P + l] += x1;
+  y[(long long)(n - 1) * P + l] = carry;
+}
+
+// Backward substitution (dgtts2) of one line: q += x, rows first..n-1 (the
+// q_r wall rows are reset instead).  Loads are issued a chunk of rows ahead
+// of the dependent recurrence.
+DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
+                      const double *__restrict__ y, long long l, long long P) {
+  constexpr int CH = 8;
+  double x1 = y[(long long)(n - 1) * P + l] * T.rd[n - 1];
+  if (n - 1 >= first) q[(long long)(n - 1) * P + l] += x1;
   double x0 = (y[(long long)(n - 2) * P + l] - T.du[n - 2] * x1) * T.rd[n - 2];
-  qz[(long long)(n - 2) * P + l] += x0;
-  for (int r = n - 3; r >= 0; r--) {
-    const long long o = (long long)r * P + l;
-    const double x = (y[o] - T.du[r] * x0 - T.du2[r] * x1) * T.rd[r];
-    qz[o] += x;
-    x1 = x0;
-    x0 = x;
+  if (n - 2 >= first) q[(long long)(n - 2) * P + l] += x0;
+  for (int r0 = n - 3; r0 >= first; r0 -= CH) {
+    double yv[CH], qv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const int r = r0 - u;
+      if (r >= first) {
+        const long long o = (long long)r * P + l;
+        yv[u] = y[o];
+        qv[u] = q[o];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const int r = r0 - u;
+      if (r >= first) {
+        const double x = (yv[u] - T.du[r] * x0 - T.du2[r] * x1) * T.rd[r];
+        q[(long long)r * P + l] = qv[u] + x;
+        x1 = x0;
+        x0 = x;
+      }
+    }
+  }
+}
+
+// which: bit 0 q_theta, bit 1 q_r, bit 2 q_z
+__global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict__ qt,
+                                                  double *__restrict__ qr,
+                                                  double *__restrict__ qz,
+                                                  const double *__restrict__ yt,
+                                                  const double *__restrict__ yr,
+                                                  const double *__restrict__ yz,
+                                                  const unsigned char *__restrict__ icyl,
+                                                  const unsigned char *__restrict__ ocyl,
+                                                  const double *__restrict__ lut,
+                                                  const double *__restrict__ lur,
+                                                  const double *__restrict__ luz, int nt,
+                                                  int which) {
+  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= g.P) return;
+  const int v = icyl[l] + 2 * ocyl[l];
+  if (which & 1) usolve_line(lu_view(lut + (size_t)v * nt, nt), g.nrm, 0, qt, yt, l, g.P);
+  if (which & 2) {
+    // SolveImpEqnUpdate_QR.F90:346-348 adds rows 2..nrm; walls reset
+    // (ImplicitAndUpdateQR.F90:278-279)
+    usolve_line(lu_view(lur, nt), g.nr, 1, qr, yr, l, g.P);
+    qr[l] = 0.0;
+    qr[(long long)(g.nr - 1) * g.P + l] = 0.0;
+  }
+  if (which & 4) usolve_line(lu_view(luz + (size_t)v * nt, nt), g.nrm, 0, qz, yz, l, g.P);
+}
+
+// ---------------------------------------------------------------------------
+// Fused substep front: explicit terms + right-hand sides + forward
+// substitution of all three components in one march along r.  A block owns
+// a 64 (theta) x 4 (z) tile of radial lines; the planes k-1, k, k+1 of
+// qt, qr, qz (with a one-point periodic halo) rotate through an LDS ring and
+// plane k+2 is fetched into registers while plane k is computed.  The
+// explicit terms never reach HBM.
+// ---------------------------------------------------------------------------
+constexpr int FT_I = 64, FT_W = FT_I + 2;
+
+struct LAcc {
+  const double *m, *c, *p;  // planes k-1, k, k+1 of one field
+  int base;                 // (row+1)*FT_W + tx+1
+  DEVI double operator()(int dk, int dj, int di) const {
+    const double *s = dk < 0 ? m : (dk > 0 ? p : c);
+    return s[base + dj * FT_W + di];
+  }
+};
+
+// Workgroups are dispatched round-robin over the XCDs; hand each XCD a
+// contiguous run of tiles so neighbouring tiles share one L2.
+DEVI int xcd_tile(int b, int nb) {
+  constexpr int NX = 8;
+  const int q = nb / NX, r = nb - q * NX;
+  const int x = b % NX, local = b / NX;
+  return x * q + (x < r ? x : r) + local;
+}
+
+// A block of 256 threads owns a 64 (theta) x 4*LPT (z) tile of radial lines,
+// LPT lines per thread (rows ty and ty+4).  UNIFORM: every wall mask is 1
+// (no-slip everywhere, the SlipBCs default), so all lines share one table
+// entry and the table pointers stay scalar.
+template <bool UNIFORM, int LPT>
+__global__ __launch_bounds__(256) void k_helm_fwd(
+    GridDev g, ImpArgs a, const double *__restrict__ qt, const double *__restrict__ qr,
+    const double *__restrict__ qz, const double *__restrict__ pr, double *__restrict__ ru1,
+    double *__restrict__ ru2, double *__restrict__ ru3, double *__restrict__ yt,
+    double *__restrict__ yr, double *__restrict__ yz, const unsigned char *__restrict__ icyl,
+    const unsigned char *__restrict__ ocyl, const double *__restrict__ lut,
+    const double *__restrict__ lur, const double *__restrict__ luz, int nt, int ntx, int ntz) {
+  constexpr int TJ = 4 * LPT, H = TJ + 2, S = FT_W * H, NE = (S + 255) / 256;
+  __shared__ double sq[3][3][S];  // field, ring slot, tile
+  __shared__ double sp[2][S];     // pressure planes k and k+1
+  const int tile = xcd_tile(blockIdx.x, ntx * ntz);
+  const int i0 = (tile % ntx) * FT_I, j0 = (tile / ntx) * TJ;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const long long P = g.P;
+  // the tile elements this thread stages per plane
+  long long off[NE];
+  bool has[NE];
+#pragma unroll
+  for (int q = 0; q < NE; q++) {
+    const int e = threadIdx.x + q * 256;
+    has[q] = e < S;
+    const int ee = has[q] ? e : 0;
+    int jj = j0 + ee / FT_W - 1, ii = i0 + ee % FT_W - 1;
+    jj = ((jj % g.nzm) + g.nzm) % g.nzm;
+    ii = ((ii % g.nthm) + g.nthm) % g.nthm;
+    off[q] = (long long)jj * g.nthm + ii;
+  }
+  // per-line state
+  bool valid[LPT];
+  long long l[LPT];
+  int base[LPT];
+  double fic[LPT], foc[LPT], h1[LPT], h2[LPT], h3[LPT], ct[LPT], cr[LPT], cz[LPT], pr_km[LPT];
+  LUView Tt[LPT], Tz[LPT];
+  const LUView Tr = lu_view(lur, nt);
+#pragma unroll
+  for (int u = 0; u < LPT; u++) {
+    const int row = ty + 4 * u;
+    const int i = i0 + tx, j = j0 + row;
+    valid[u] = (i < g.nthm) && (j < g.nzm);
+    l[u] = valid[u] ? (long long)j * g.nthm + i : 0;
+    base[u] = (row + 1) * FT_W + tx + 1;
+    const int ic = (UNIFORM || !valid[u]) ? 1 : icyl[l[u]];
+    const int oc = (UNIFORM || !valid[u]) ? 1 : ocyl[l[u]];
+    fic[u] = (double)ic;
+    foc[u] = (double)oc;
+    const int v = UNIFORM ? 3 : ic + 2 * oc;
+    Tt[u] = lu_view(lut + (size_t)v * nt, nt);
+    Tz[u] = lu_view(luz + (size_t)v * nt, nt);
+    h1[u] = valid[u] ? ru1[l[u]] : 0.0;
+    h2[u] = 0.0;
+    h3[u] = valid[u] ? ru3[l[u]] : 0.0;
+    ct[u] = cr[u] = cz[u] = pr_km[u] = 0.0;
+  }
+  // planes 1 and 2 of the velocity, plane 1 of the pressure
+#pragma unroll
+  for (int q = 0; q < NE; q++) {
+    if (!has[q]) continue;
+    const int e = threadIdx.x + q * 256;
+    sq[0][1][e] = qt[off[q]];
+    sq[1][1][e] = qr[off[q]];
+    sq[2][1][e] = qz[off[q]];
+    sq[0][2][e] = qt[P + off[q]];
+    sq[1][2][e] = qr[P + off[q]];
+    sq[2][2][e] = qz[P + off[q]];
+    sp[1][e] = pr[off[q]];
+  }
+  __syncthreads();
+  const int nrm = g.nrm;
+  double yv;
+
+  for (int k = 1; k <= nrm; k++) {
+    // prefetch plane k+2 of the velocity, plane k+1 of the pressure and of
+    // the RK histories
+    double nq[3][NE], np[NE], n1[LPT], n2[LPT], n3[LPT];
+    const bool pf = (k + 2 <= g.nr), pfp = (k + 1 <= nrm);
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      if (has[q] && pf) {
+        const long long o = (long long)(k + 1) * P + off[q];
+        nq[0][q] = qt[o];
+        nq[1][q] = qr[o];
+        nq[2][q] = qz[o];
+      }
+      if (has[q] && pfp) np[q] = pr[(long long)k * P + off[q]];
+    }
+#pragma unroll
+    for (int u = 0; u < LPT; u++) {
+      n1[u] = n2[u] = n3[u] = 0.0;
+      if (valid[u] && pfp) {
+        const long long on = (long long)k * P + l[u];
+        n1[u] = ru1[on];
+        n2[u] = ru2[on];
+        n3[u] = ru3[on];
+      }
+    }
+    const int sm = (k + 2) % 3, sc = k % 3, sn = (k + 1) % 3;  // planes k-1, k, k+1
+    const double *spk = sp[k & 1];
+#pragma unroll
+    for (int u = 0; u < LPT; u++) {
+      if (!valid[u]) continue;
+      LAcc T, R, Z;
+      T.m = sq[0][sm]; T.c = sq[0][sc]; T.p = sq[0][sn]; T.base = base[u];
+      R.m = sq[1][sm]; R.c = sq[1][sc]; R.p = sq[1][sn]; R.base = base[u];
+      Z.m = sq[2][sm]; Z.c = sq[2][sc]; Z.p = sq[2][sn]; Z.base = base[u];
+      const double pr_c = spk[base[u]], pr_im = spk[base[u] - 1], pr_jm = spk[base[u] - FT_W];
+      const long long o = (long long)(k - 1) * P + l[u];
+      // q_theta
+      {
+        const double dqv = explicit_qt_g(g, T, R, Z, k);
+        const double qm = (k > 1) ? T(-1, 0, 0) : 0.0, qp = (k < nrm) ? T(1, 0, 0) : 0.0;
+        const double r = rhs_qt(g, a, k, qm, T(0, 0, 0), qp, pr_c, pr_im, dqv, h1[u], fic[u], foc[u]);
+        ru1[o] = dqv;
+        if (k == 1) {
+          ct[u] = (r + a.bc1) * Tt[u].sc[0];
+        } else {
+          lsolve_step(Tt[u], k - 2, r * Tt[u].sc[k - 1], ct[u], yv);
+          yt[o - P] = yv;
+        }
+      }
+      // q_r (rows 2..nrm; row 1 is the wall with zero right-hand side)
+      if (k >= 2) {
+        const double dphv = explicit_qr_g(g, T, R, Z, k);
+        const double r = rhs_qr(g, a, k, R(-1, 0, 0), R(0, 0, 0), R(1, 0, 0), pr_c, pr_km[u], dphv, h2[u]);
+        ru2[o] = dphv;
+        lsolve_step(Tr, k - 2, r * Tr.sc[k - 1], cr[u], yv);
+        yr[o - P] = yv;
+      }
+      // q_z
+      {
+        const double qcv = explicit_qz_g(g, T, R, Z, k);
+        const double qm = (k > 1) ? Z(-1, 0, 0) : 0.0, qp = (k < nrm) ? Z(1, 0, 0) : 0.0;
+        const double r = rhs_qz(g, a, k, qm, Z(0, 0, 0), qp, pr_c, pr_jm, qcv, h3[u], fic[u], foc[u]);
+        ru3[o] = qcv;
+        if (k == 1) {
+          cz[u] = r * Tz[u].sc[0];
+        } else {
+          lsolve_step(Tz[u], k - 2, r * Tz[u].sc[k - 1], cz[u], yv);
+          yz[o - P] = yv;
+        }
+      }
+      pr_km[u] = pr_c;
+    }
+#pragma unroll
+    for (int u = 0; u < LPT; u++) {
+      h1[u] = n1[u];
+      h2[u] = n2[u];
+      h3[u] = n3[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int e = threadIdx.x + q * 256;
+      if (has[q] && pf) {
+        const int s2 = (k + 2) % 3;
+        sq[0][s2][e] = nq[0][q];
+        sq[1][s2][e] = nq[1][q];
+        sq[2][s2][e] = nq[2][q];
+      }
+      if (has[q] && pfp) sp[(k + 1) & 1][e] = np[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < LPT; u++) {
+    if (!valid[u]) continue;
+    yt[(long long)(nrm - 1) * P + l[u]] = ct[u];
+    yz[(long long)(nrm - 1) * P + l[u]] = cz[u];
+    double c2 = cr[u];
+    lsolve_step(Tr, g.nr - 2, 0.0, c2, yv);  // wall row nr
+    yr[(long long)(g.nr - 2) * P + l[u]] = yv;
+    yr[(long long)(g.nr - 1) * P + l[u]] = c2;
   }
 }
 
@@ -434,12 +678,27 @@ static const double *lu_slot(const tc_ctx *c, int comp) {
 
 static unsigned line_blocks(const tc_ctx *c) { return (unsigned)((c->P + 255) / 256); }
 
+static void launch_bwd(tc_ctx *c, int which, const char *name) {
+  timer_begin(c, name);
+  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_DQ],
+                     c->f[TC_QCAP], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
+                     lu_slot(c, LU_QZ), c->lu_nt, which);
+  timer_end(c);
+}
+
+// Routine-level entries: right-hand side + forward substitution into the rhs
+// field, then the backward substitution.
 void launch_implicit_qt(tc_ctx *c) {
   timer_begin(c, "implicit_qt");
   hipLaunchKernelGGL(k_implicit_qt, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
                      imp_args(c), c->f[TC_QT], c->f[TC_PR], c->f[TC_DQ], c->f[TC_RU1],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), c->lu_nt);
   timer_end(c);
+  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_RHS],
+                     c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
+                     lu_slot(c, LU_QZ), c->lu_nt, 1);
 }
 void launch_implicit_qr(tc_ctx *c) {
   timer_begin(c, "implicit_qr");
@@ -447,6 +706,10 @@ void launch_implicit_qr(tc_ctx *c) {
                      imp_args(c), c->f[TC_QR], c->f[TC_PR], c->f[TC_DPH], c->f[TC_RU2],
                      c->f[TC_RHS], lu_slot(c, LU_QR), c->lu_nt);
   timer_end(c);
+  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_RHS],
+                     c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
+                     lu_slot(c, LU_QZ), c->lu_nt, 2);
 }
 void launch_implicit_qz(tc_ctx *c) {
   timer_begin(c, "implicit_qz");
@@ -454,6 +717,61 @@ void launch_implicit_qz(tc_ctx *c) {
                      imp_args(c), c->f[TC_QZ], c->f[TC_PR], c->f[TC_QCAP], c->f[TC_RU3],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QZ), c->lu_nt);
   timer_end(c);
+  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_RHS],
+                     c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
+                     lu_slot(c, LU_QZ), c->lu_nt, 4);
+}
+
+// Fused path: y_theta -> rhs, y_r -> dq, y_z -> qcap (the explicit terms
+// stay in registers, so those fields serve as scratch).
+template <bool U, int LPT>
+static void launch_fwd(tc_ctx *c) {
+  const int ntx = (c->nthm + FT_I - 1) / FT_I, ntz = (c->nzm + 4 * LPT - 1) / (4 * LPT);
+  hipLaunchKernelGGL((k_helm_fwd<U, LPT>), dim3((unsigned)(ntx * ntz)), dim3(256), 0, c->stream,
+                     c->gd, imp_args(c), c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR],
+                     c->f[TC_RU1], c->f[TC_RU2], c->f[TC_RU3], c->f[TC_RHS], c->f[TC_DQ],
+                     c->f[TC_QCAP], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
+                     lu_slot(c, LU_QZ), c->lu_nt, ntx, ntz);
+}
+
+// Lines per thread of the fused kernel: the variant that needs the fewest
+// rounds of resident blocks (one line per thread on a tie).
+int choose_helm_lpt(const tc_ctx *c) {
+  int dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (cus <= 0) return 1;
+  double best = 1e30;
+  int pick = 1;
+  for (int lpt = 1; lpt <= 2; lpt++) {
+    int per_cu = 0;
+    hipError_t e = (lpt == 1)
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 1>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 2>, 256, 0);
+    if (e != hipSuccess || per_cu <= 0) continue;
+    const long long tiles = (long long)((c->nthm + FT_I - 1) / FT_I) * ((c->nzm + 4 * lpt - 1) / (4 * lpt));
+    const long long slots = (long long)per_cu * cus;
+    const long long rounds = (tiles + slots - 1) / slots;
+    // each round costs one march; a block with two lines per thread works ~1.6x longer
+    const double cost = (double)rounds * (lpt == 1 ? 1.0 : 1.6);
+    if (cost < best - 1e-9) {
+      best = cost;
+      pick = lpt;
+    }
+  }
+  return pick;
+}
+
+void launch_helmholtz_fused(tc_ctx *c) {
+  timer_begin(c, "helm_fwd");
+  if (c->masks_uniform_one) {
+    if (c->helm_lpt == 2) launch_fwd<true, 2>(c); else launch_fwd<true, 1>(c);
+  } else {
+    if (c->helm_lpt == 2) launch_fwd<false, 2>(c); else launch_fwd<false, 1>(c);
+  }
+  timer_end(c);
+  launch_bwd(c, 7, "helm_bwd");
 }
 
 // ---------------------------------------------------------------------------
@@ -586,11 +904,10 @@ void launch_zero_wall_qr(tc_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// Reductions: fixed grid of RED_BLOCKS blocks, per-block partials, then one
-// block combines the partials in a fixed order (deterministic sums).
+// Reductions: per-block partials, then one block per value combines them in
+// a fixed order, so sums are deterministic run to run.
 // ---------------------------------------------------------------------------
 
-constexpr int RED_BLOCKS = 1024;
 constexpr int RED_THREADS = 256;
 
 template <int NV>
@@ -617,36 +934,54 @@ DEVI void block_reduce(double (&v)[NV], const int *ops /*0 sum, 1 max*/, double 
   }
 }
 
-// CalcMaxCFL.F90:30-43 and CheckDivergence.F90:77-89 (WHICH 0 / 1)
+// Point reductions march along r: thread = (j, i) column, k = 1..nrm, so the
+// qr(k+1) plane of one step is the qr(k) plane of the next.
+struct Col {
+  long long l;
+  long long lip, ljp;  // in-plane offsets of (j, i+1) and (j+1, i)
+  bool ok;
+};
+DEVI Col column(const GridDev &g) {
+  Col c;
+  c.l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  c.ok = c.l < g.P;
+  const long long l = c.ok ? c.l : 0;
+  const int j = (int)(l / g.nthm), i = (int)(l - (long long)j * g.nthm);
+  c.lip = (long long)j * g.nthm + wrap_p(i, g.nthm);
+  c.ljp = (long long)wrap_p(j, g.nzm) * g.nthm + i;
+  return c;
+}
+
+// CalcMaxCFL.F90:30-43 (WHICH 0) and CheckDivergence.F90:77-89 (WHICH 1)
 template <int WHICH>
 __global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const double *__restrict__ qt,
                                                          const double *__restrict__ qr,
                                                          const double *__restrict__ qz,
                                                          double *__restrict__ part) {
   double m[1] = {0.0};
-  const long long total = (long long)g.nrm * g.P;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int kc = (int)(q / g.P) + 1;
-    const long long r = q - (long long)(kc - 1) * g.P;
-    const int j = (int)(r / g.nthm), i = (int)(r - (long long)j * g.nthm);
-    double val;
-    if (WHICH == 0) {
-      const int ip = wrap_p(i, g.nthm), jp = wrap_p(j, g.nzm);
-      const long long o = q;
-      val = (fabs((qt[IX(g, kc, j, ip)] + qt[o]) * 0.5 * g.dxt * g.usrm[kc]) +
-             fabs((qr[o + g.P] + qr[o]) * 0.5 * g.udx2rm[kc]) +
-             fabs((qz[IX(g, kc, jp, i)] + qz[o]) * 0.5 * g.dxz));
-    } else {
-      val = fabs(divergence_at(g, qt, qr, qz, kc, j, i));
+  const Col c = column(g);
+  if (c.ok) {
+    double r_c = qr[c.l];
+    for (int kc = 1; kc <= g.nrm; kc++) {
+      const long long b = (long long)(kc - 1) * g.P;
+      const double t_c = qt[b + c.l], t_ip = qt[b + c.lip];
+      const double r_kp = qr[b + g.P + c.l];
+      const double z_c = qz[b + c.l], z_jp = qz[b + c.ljp];
+      double val;
+      if (WHICH == 0)
+        val = (fabs((t_ip + t_c) * 0.5 * g.dxt * g.usrm[kc]) + fabs((r_kp + r_c) * 0.5 * g.udx2rm[kc]) +
+               fabs((z_jp + z_c) * 0.5 * g.dxz));
+      else
+        val = fabs(((t_ip - t_c) * g.dxt + (r_kp - r_c) * g.udx2m[kc]) * g.usrm[kc] + (z_jp - z_c) * g.dxz);
+      m[0] = fmax(m[0], val);
+      r_c = r_kp;
     }
-    m[0] = fmax(m[0], val);
   }
   const int ops[1] = {1};
   block_reduce<1>(m, ops, part + blockIdx.x);
 }
 
-// CalcTorque.F90:131-140 masked wall sums and CalcVelocityStats.F90:29-48
+// CalcTorque.F90:131-140 masked wall sums
 __global__ __launch_bounds__(RED_THREADS) void k_torque(GridDev g, const double *__restrict__ qt,
                                                        const unsigned char *__restrict__ icyl,
                                                        const unsigned char *__restrict__ ocyl,
@@ -655,75 +990,90 @@ __global__ __launch_bounds__(RED_THREADS) void k_torque(GridDev g, const double 
                                                        double *__restrict__ part) {
   double s[2] = {0.0, 0.0};
   const int nrm = g.nrm, nr = g.nr;
-  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < g.P;
-       l += (long long)gridDim.x * blockDim.x) {
+  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < g.P) {
     const double grtlow = ((qt[l + g.P] * g.usrm[2] + qt[l] * g.usrm[1]) * 0.5 - virotint * g.usrc[1]) * del;
-    s[0] += grtlow * (double)icyl[l];
+    s[0] = grtlow * (double)icyl[l];
     const double grtup = (virotext * g.usrc[nr] - (qt[(long long)(nrm - 2) * g.P + l] * g.usrm[nrm - 1] +
                                                    qt[(long long)(nrm - 1) * g.P + l] * g.usrm[nrm]) * 0.5) * deln;
-    s[1] += grtup * (double)ocyl[l];
+    s[1] = grtup * (double)ocyl[l];
   }
   const int ops[2] = {0, 0};
   block_reduce<2>(s, ops, part + 2 * blockIdx.x);
 }
 
+// CalcVelocityStats.F90:29-48
 __global__ __launch_bounds__(RED_THREADS) void k_velstats(GridDev g, const double *__restrict__ qt,
                                                          const double *__restrict__ qr,
                                                          const double *__restrict__ qz,
                                                          double pi, double *__restrict__ part) {
   double v[6] = {-100.0, -100.0, -100.0, 0.0, 0.0, 0.0};
-  const long long total = (long long)g.nrm * g.P;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int kc = (int)(q / g.P) + 1;
-    const long long r = q - (long long)(kc - 1) * g.P;
-    const int j = (int)(r / g.nthm), i = (int)(r - (long long)j * g.nthm);
-    const double a = qt[q], b = qr[q], cz = qz[q];
-    v[0] = fmax(v[0], fabs(a));
-    v[1] = fmax(v[1], fabs(b));
-    v[2] = fmax(v[2], fabs(cz));
-    const double t = a + qt[IX(g, kc, j, wrap_p(i, g.nthm))];
-    v[3] += t * t * g.pvol[kc] * pi * 0.25;
-    const double u = b * g.usrc[kc] + qr[q + g.P] * g.usrc[kc + 1];
-    v[4] += u * u * g.pvol[kc] * pi * 0.25;
-    const double w = cz + qz[IX(g, kc, wrap_p(j, g.nzm), i)];
-    v[5] += w * w * g.pvol[kc] * pi * 0.25;
+  const Col c = column(g);
+  if (c.ok) {
+    double r_c = qr[c.l];
+    for (int kc = 1; kc <= g.nrm; kc++) {
+      const long long b = (long long)(kc - 1) * g.P;
+      const double a = qt[b + c.l], cz = qz[b + c.l];
+      const double r_kp = qr[b + g.P + c.l];
+      v[0] = fmax(v[0], fabs(a));
+      v[1] = fmax(v[1], fabs(r_c));
+      v[2] = fmax(v[2], fabs(cz));
+      const double t = a + qt[b + c.lip];
+      v[3] += t * t * g.pvol[kc] * pi * 0.25;
+      const double u = r_c * g.usrc[kc] + r_kp * g.usrc[kc + 1];
+      v[4] += u * u * g.pvol[kc] * pi * 0.25;
+      const double w = cz + qz[b + c.ljp];
+      v[5] += w * w * g.pvol[kc] * pi * 0.25;
+      r_c = r_kp;
+    }
   }
   const int ops[6] = {1, 1, 1, 0, 0, 0};
   block_reduce<6>(v, ops, part + 6 * blockIdx.x);
 }
 
-// combine nv values per block over nb blocks, in block order
-__global__ void k_combine(const double *__restrict__ part, int nb, int nv, unsigned opmask,
-                          double *__restrict__ out) {
-  const int q = threadIdx.x;
-  if (q >= nv) return;
+// combine: block q reduces value q over the nb partials (fixed order: strided
+// per-thread sums, then a fixed tree)
+__global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part, int nb, int nv,
+                                                 unsigned opmask, double *__restrict__ out) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x;
   const bool mx = (opmask >> q) & 1u;
-  double x = part[q];
-  for (int b = 1; b < nb; b++) {
+  double x = mx ? -1.0e308 : 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
     const double y = part[(size_t)b * nv + q];
     x = mx ? fmax(x, y) : x + y;
   }
-  out[q] = x;
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double y = sh[threadIdx.x + w];
+      sh[threadIdx.x] = mx ? fmax(sh[threadIdx.x], y) : sh[threadIdx.x] + y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = sh[0];
 }
 
+static unsigned red_blocks(const tc_ctx *c) { return (unsigned)((c->P + RED_THREADS - 1) / RED_THREADS); }
+
 static void finish_reduction(tc_ctx *c, int nv, unsigned opmask) {
-  hipLaunchKernelGGL(k_combine, dim3(1), dim3(64), 0, c->stream, c->red, RED_BLOCKS, nv, opmask,
-                     c->red_out);
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)nv), dim3(256), 0, c->stream, c->red, (int)red_blocks(c),
+                     nv, opmask, c->red_out);
   hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost,
                  c->stream);
 }
 
 void launch_max_cfl(tc_ctx *c) {
   timer_begin(c, "max_cfl");
-  hipLaunchKernelGGL(k_pointmax<0>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_pointmax<0>, dim3(red_blocks(c)), dim3(RED_THREADS), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->red);
   finish_reduction(c, 1, 1u);
   timer_end(c);
 }
 void launch_max_div(tc_ctx *c) {
   timer_begin(c, "max_div");
-  hipLaunchKernelGGL(k_pointmax<1>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_pointmax<1>, dim3(red_blocks(c)), dim3(RED_THREADS), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->red);
   finish_reduction(c, 1, 1u);
   timer_end(c);
@@ -731,15 +1081,90 @@ void launch_max_div(tc_ctx *c) {
 void launch_torque(tc_ctx *c) {
   const double del = -1.0 / (c->g.rc[2] - c->g.rc[1]);
   const double deln = -1.0 / (c->g.rc[c->nr] - c->g.rc[c->nrm]);
-  hipLaunchKernelGGL(k_torque, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_torque, dim3(red_blocks(c)), dim3(RED_THREADS), 0, c->stream, c->gd,
                      c->f[TC_QT], c->mask, c->mask + c->P, c->p.virotint, c->p.virotext, del,
                      deln, c->red);
   finish_reduction(c, 2, 0u);
 }
 void launch_velocity_stats(tc_ctx *c) {
-  hipLaunchKernelGGL(k_velstats, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_velstats, dim3(red_blocks(c)), dim3(RED_THREADS), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->g.pi, c->red);
   finish_reduction(c, 6, 7u);
+}
+
+// One pass over the velocity for the logging step of main.F90:180-186:
+// CalcMaxCFL (CalcMaxCFL.F90:36-39), CheckDivergence (CheckDivergence.F90:83-86),
+// CalcVelocityStats (CalcVelocityStats.F90:35-44) and CalcTorque
+// (CalcTorque.F90:133-138).  v = cfl, |div|, vmax1..3, rms sums 1..3,
+// torque sums inner/outer.
+__global__ __launch_bounds__(RED_THREADS) void k_diagnostics(
+    GridDev g, const double *__restrict__ qt, const double *__restrict__ qr,
+    const double *__restrict__ qz, const unsigned char *__restrict__ icyl,
+    const unsigned char *__restrict__ ocyl, double virotint, double virotext, double del,
+    double deln, double pi, double *__restrict__ part) {
+  double v[10] = {0.0, 0.0, -100.0, -100.0, -100.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const Col c = column(g);
+  if (c.ok) {
+    double r_c = qr[c.l], t_prev = 0.0;
+    for (int kc = 1; kc <= g.nrm; kc++) {
+      const long long b = (long long)(kc - 1) * g.P;
+      const double t_c = qt[b + c.l], t_ip = qt[b + c.lip];
+      const double r_kp = qr[b + g.P + c.l];
+      const double z_c = qz[b + c.l], z_jp = qz[b + c.ljp];
+      const double cfl = (fabs((t_ip + t_c) * 0.5 * g.dxt * g.usrm[kc]) +
+                          fabs((r_kp + r_c) * 0.5 * g.udx2rm[kc]) + fabs((z_jp + z_c) * 0.5 * g.dxz));
+      const double dv = ((t_ip - t_c) * g.dxt + (r_kp - r_c) * g.udx2m[kc]) * g.usrm[kc] + (z_jp - z_c) * g.dxz;
+      v[0] = fmax(v[0], cfl);
+      v[1] = fmax(v[1], fabs(dv));
+      v[2] = fmax(v[2], fabs(t_c));
+      v[3] = fmax(v[3], fabs(r_c));
+      v[4] = fmax(v[4], fabs(z_c));
+      const double t = t_c + t_ip;
+      v[5] += t * t * g.pvol[kc] * pi * 0.25;
+      const double u = r_c * g.usrc[kc] + r_kp * g.usrc[kc + 1];
+      v[6] += u * u * g.pvol[kc] * pi * 0.25;
+      const double w = z_c + z_jp;
+      v[7] += w * w * g.pvol[kc] * pi * 0.25;
+      if (kc == 2) {
+        const double grtlow = ((t_c * g.usrm[2] + t_prev * g.usrm[1]) * 0.5 - virotint * g.usrc[1]) * del;
+        v[8] += grtlow * (double)icyl[c.l];
+      }
+      if (kc == g.nrm) {
+        const double grtup = (virotext * g.usrc[g.nr] - (t_prev * g.usrm[g.nrm - 1] + t_c * g.usrm[g.nrm]) * 0.5) * deln;
+        v[9] += grtup * (double)ocyl[c.l];
+      }
+      t_prev = t_c;
+      r_c = r_kp;
+    }
+  }
+  const int ops[10] = {1, 1, 1, 1, 1, 0, 0, 0, 0, 0};
+  block_reduce<10>(v, ops, part + 10 * blockIdx.x);
+}
+
+void launch_diagnostics(tc_ctx *c) {
+  const double del = -1.0 / (c->g.rc[2] - c->g.rc[1]);
+  const double deln = -1.0 / (c->g.rc[c->nr] - c->g.rc[c->nrm]);
+  timer_begin(c, "diagnostics");
+  hipLaunchKernelGGL(k_diagnostics, dim3(red_blocks(c)), dim3(RED_THREADS), 0, c->stream, c->gd,
+                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->mask, c->mask + c->P,
+                     c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red);
+  finish_reduction(c, 10, 0x1Fu);
+  timer_end(c);
+}
+
+// Plain streaming copy of one field (nr planes): the reference point for
+// converting hardware byte counters into bytes per launch.
+__global__ __launch_bounds__(256) void k_copy(const double *__restrict__ src, double *__restrict__ dst,
+                                              long long n) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+
+void launch_copy(tc_ctx *c, int dst, int src) {
+  timer_begin(c, "copy");
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, c->f[src], c->f[dst], c->N);
+  timer_end(c);
 }
 
 }  // namespace tc

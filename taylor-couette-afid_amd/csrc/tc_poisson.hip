@@ -99,18 +99,16 @@ DEVI void dft<8>(double2 *v, double sg) {
   }
 }
 
-template <int R>
-constexpr int maxb() {
-  return (16 / R) > 0 ? 16 / R : 1;
-}
-
-// One Stockham pass of radix R over a line of n points in LDS (in place:
-// read all inputs to registers, barrier, write, barrier).  Ns is the product
-// of the radices of the previous passes.  All threads of the block must call.
-template <int R>
-DEVI void fft_pass(double2 *buf, int n, int Ns, int t, int T, const double2 *__restrict__ tw,
-                   double sg) {
-  constexpr int MB = maxb<R>();
+// One Stockham pass of radix R over a line of n points in LDS, element e at
+// buf[e*stride].  In place: every thread reads the inputs of its butterflies
+// into registers, barrier, writes, barrier.  Ns is the product of the radices
+// of the earlier passes.  ME bounds the points a thread holds per pass
+// (n <= ME*T), so the register arrays have a compile-time size.  All threads
+// of the block must call.
+template <int R, int ME>
+DEVI void fft_pass(double2 *buf, int stride, int n, int Ns, int t, int T,
+                   const double2 *__restrict__ tw, double sg) {
+  constexpr int MB = (ME + R - 1) / R;
   const int nb = n / R;
   const int twstep = n / (Ns * R);
   double2 v[MB][R];
@@ -121,7 +119,7 @@ DEVI void fft_pass(double2 *buf, int n, int Ns, int t, int T, const double2 *__r
       const int k = b % Ns;
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        double2 x = buf[b + r * nb];
+        double2 x = buf[(b + r * nb) * stride];
         if (r > 0 && k > 0) {
           double2 w = tw[r * k * twstep];
           w.y = (sg < 0) ? w.y : -w.y;
@@ -140,30 +138,33 @@ DEVI void fft_pass(double2 *buf, int n, int Ns, int t, int T, const double2 *__r
       const int k = b % Ns;
       const int base = (b / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[base + r * Ns] = v[q][r];
+      for (int r = 0; r < R; r++) buf[(base + r * Ns) * stride] = v[q][r];
     }
   }
   __syncthreads();
 }
 
-DEVI void fft_line(double2 *buf, const FFTPlan &pl, int t, int T, const double2 *__restrict__ tw,
-                   double sg) {
+template <int ME>
+DEVI void fft_line(double2 *buf, int stride, const FFTPlan &pl, int t, int T,
+                   const double2 *__restrict__ tw, double sg) {
   int Ns = 1;
   for (int s = 0; s < pl.nst; s++) {
     const int R = pl.radix[s];
     switch (R) {
-      case 8: fft_pass<8>(buf, pl.n, Ns, t, T, tw, sg); break;
-      case 4: fft_pass<4>(buf, pl.n, Ns, t, T, tw, sg); break;
-      case 2: fft_pass<2>(buf, pl.n, Ns, t, T, tw, sg); break;
-      case 3: fft_pass<3>(buf, pl.n, Ns, t, T, tw, sg); break;
-      default: fft_pass<5>(buf, pl.n, Ns, t, T, tw, sg); break;
+      case 8: fft_pass<8, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
+      case 4: fft_pass<4, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
+      case 2: fft_pass<2, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
+      case 3: fft_pass<3, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
+      default: fft_pass<5, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
     }
     Ns *= R;
   }
 }
 
-// r2c along theta for row pairs.  row r of the real input is at in + r*nthm,
-// row r of the half spectrum at spec + r*nth2.
+// r2c along theta for row pairs: rows 2p and 2p+1 of the real input become
+// the real and imaginary parts of one complex FFT.  Row r of the real input
+// is at in + r*nthm, row r of the half spectrum at spec + r*nth2.
+template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict__ in,
                                                        double2 *__restrict__ spec, FFTPlan pl,
                                                        const double2 *__restrict__ tw,
@@ -174,11 +175,11 @@ __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict_
   const long long pair = (long long)blockIdx.x * L + ln;
   const long long r0 = 2 * pair, r1 = r0 + 1;
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
-  double2 *buf = lds + ln * (n + 1);
+  double2 *buf = lds + ln * n;
   const double *a = in + r0 * n, *b = in + r1 * n;
   for (int e = t; e < n; e += T) buf[e] = make_double2(v0 ? a[e] : 0.0, v1 ? b[e] : 0.0);
   __syncthreads();
-  fft_line(buf, pl, t, T, tw, -1.0);
+  fft_line<ME>(buf, 1, pl, t, T, tw, -1.0);
   for (int k = t; k < nth2; k += T) {
     const double2 z = buf[k];
     const double2 zc = buf[(n - k) % n];
@@ -192,6 +193,7 @@ __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict_
 
 // c2r along theta for row pairs; the imaginary parts of ktheta = 0 and
 // (n even) ktheta = n/2 are ignored, as a c2r transform does.
+template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict__ spec,
                                                        double *__restrict__ out, FFTPlan pl,
                                                        const double2 *__restrict__ tw,
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
   const long long pair = (long long)blockIdx.x * L + ln;
   const long long r0 = 2 * pair, r1 = r0 + 1;
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
-  double2 *buf = lds + ln * (n + 1);
+  double2 *buf = lds + ln * n;
   for (int k = t; k < n; k += T) {
     const bool lo = (k < nth2);
     const int kk = lo ? k : n - k;
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
     buf[k] = make_double2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  fft_line(buf, pl, t, T, tw, 1.0);
+  fft_line<ME>(buf, 1, pl, t, T, tw, 1.0);
   for (int e = t; e < n; e += T) {
     const double2 z = buf[e];
     if (v0) out[r0 * n + e] = z.x;
@@ -227,24 +229,27 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
   }
 }
 
-// c2c along z for a tile of pl.lines ktheta columns of one r-plane.
+// c2c along z for a tile of pl.lines ktheta columns of one r-plane.  The LDS
+// tile keeps the global row-major order [z][column]; lane = column fastest,
+// so global and LDS accesses are both contiguous across lanes.
+template <int ME>
 __global__ __launch_bounds__(256) void k_fft_z(double2 *__restrict__ spec, FFTPlan pl,
                                                const double2 *__restrict__ tw, int nth2,
                                                double sg) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = 256 / L;
-  const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
+  const int c = threadIdx.x % L, t = threadIdx.x / L;
   const int c0 = blockIdx.x * L;
   double2 *plane = spec + (long long)blockIdx.y * n * nth2;
   for (int e = threadIdx.x; e < L * n; e += 256) {
-    const int z = e / L, c = e - z * L;
-    lds[c * (n + 1) + z] = (c0 + c < nth2) ? plane[(long long)z * nth2 + c0 + c] : make_double2(0.0, 0.0);
+    const int z = e / L, cc = e - z * L;
+    lds[e] = (c0 + cc < nth2) ? plane[(long long)z * nth2 + c0 + cc] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_line(lds + ln * (n + 1), pl, t, T, tw, sg);
+  fft_line<ME>(lds + c, L, pl, t, T, tw, sg);
   for (int e = threadIdx.x; e < L * n; e += 256) {
-    const int z = e / L, c = e - z * L;
-    if (c0 + c < nth2) plane[(long long)z * nth2 + c0 + c] = lds[c * (n + 1) + z];
+    const int z = e / L, cc = e - z * L;
+    if (c0 + cc < nth2) plane[(long long)z * nth2 + c0 + cc] = lds[e];
   }
 }
 
@@ -256,6 +261,7 @@ __global__ __launch_bounds__(256) void k_fft_z(double2 *__restrict__ spec, FFTPl
 __global__ __launch_bounds__(256) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
                                                          double *__restrict__ cpw,
                                                          double inv_norm) {
+  constexpr int CH = 8;  // rows loaded ahead of the recurrence
   const long long M = (long long)g.nzm * g.nth2;
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
@@ -270,32 +276,55 @@ __global__ __launch_bounds__(256) void k_poisson_tridiag(GridDev g, double2 *__r
   double2 dp = make_double2(d.x * inv_norm * rden, d.y * inv_norm * rden);
   cpw[m] = cp;
   spec[m] = dp;
-  for (int k = 2; k <= nn; k++) {
-    const long long o = (long long)(k - 1) * M + m;
-    const double a = g.amph[k];
-    b = g.acph[k] - sz - st * (g.usrm[k] * g.usrm[k]);
-    rden = 1.0 / (b - a * cp);
-    d = spec[o];
-    dp = make_double2((d.x * inv_norm - a * dp.x) * rden, (d.y * inv_norm - a * dp.y) * rden);
-    spec[o] = dp;
-    if (k < nn) {
-      cp = g.apph[k] * rden;
-      cpw[o] = cp;
+  for (int k0 = 2; k0 <= nn; k0 += CH) {
+    double2 dv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; u++)
+      if (k0 + u <= nn) dv[u] = spec[(long long)(k0 + u - 1) * M + m];
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const int k = k0 + u;
+      if (k <= nn) {
+        const long long o = (long long)(k - 1) * M + m;
+        const double a = g.amph[k];
+        b = g.acph[k] - sz - st * (g.usrm[k] * g.usrm[k]);
+        rden = 1.0 / (b - a * cp);
+        dp = make_double2((dv[u].x * inv_norm - a * dp.x) * rden, (dv[u].y * inv_norm - a * dp.y) * rden);
+        spec[o] = dp;
+        if (k < nn) {
+          cp = g.apph[k] * rden;
+          cpw[o] = cp;
+        }
+      }
     }
   }
   // back substitution
   double2 x = dp;
-  for (int k = nn - 1; k >= 1; k--) {
-    const long long o = (long long)(k - 1) * M + m;
-    const double c = cpw[o];
-    const double2 y = spec[o];
-    x = make_double2(y.x - c * x.x, y.y - c * x.y);
-    spec[o] = x;
+  for (int k0 = nn - 1; k0 >= 1; k0 -= CH) {
+    double2 yv[CH];
+    double cv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const int k = k0 - u;
+      if (k >= 1) {
+        const long long o = (long long)(k - 1) * M + m;
+        yv[u] = spec[o];
+        cv[u] = cpw[o];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const int k = k0 - u;
+      if (k >= 1) {
+        x = make_double2(yv[u].x - cv[u] * x.x, yv[u].y - cv[u] * x.y);
+        spec[(long long)(k - 1) * M + m] = x;
+      }
+    }
   }
   if (m == 0) spec[(long long)(g.nrm - 1) * M] = make_double2(0.0, 0.0);
 }
 
-std::string make_fft_plan(int n, FFTPlan &pl, int max_lines) {
+std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes) {
   pl = FFTPlan();
   pl.n = n;
   int m = n;
@@ -305,20 +334,24 @@ std::string make_fft_plan(int n, FFTPlan &pl, int max_lines) {
   while (m % 3 == 0 && m > 1) { pl.radix[pl.nst++] = 3; m /= 3; }
   while (m % 5 == 0 && m > 1) { pl.radix[pl.nst++] = 5; m /= 5; }
   if (m != 1) return "FFT length " + std::to_string(n) + " has a prime factor other than 2, 3, 5";
-  // lines per 256-thread block: every pass must fit maxb(R) butterflies per
-  // thread, and the LDS tile (lines*(n+1) complex) must fit 64 KiB
-  int L = max_lines;
-  auto fits = [&](int lines) {
-    const int T = 256 / lines;
-    for (int s = 0; s < pl.nst; s++) {
-      const int R = pl.radix[s];
-      const int mb = (16 / R) > 0 ? 16 / R : 1;
-      if (n / R > mb * T) return false;
+  // lines per 256-thread block: as many as the LDS budget allows (at most
+  // 64); then the points per thread per pass, ME, from {4, 8, 16, 32}
+  int L = 64;
+  while (L > 1 && (long long)L * n * 16 > lds_bytes) L /= 2;
+  for (;;) {
+    const int T = 256 / L;
+    const int need = (n + T - 1) / T;
+    int me = 4;
+    while (me < need) me *= 2;
+    if (me <= 32) {
+      pl.me = me;
+      break;
     }
-    return (long long)lines * (n + 1) * 16 <= 65536;
-  };
-  while (L > 1 && !fits(L)) L /= 2;
-  if (!fits(L)) return "FFT length " + std::to_string(n) + " is too long for one block";
+    if (L == 1) return "FFT length " + std::to_string(n) + " is too long for one block";
+    L /= 2;
+  }
+  if ((long long)L * n * 16 > lds_bytes)
+    return "FFT length " + std::to_string(n) + " does not fit the LDS budget";
   pl.lines = L;
   return std::string();
 }
@@ -333,33 +366,67 @@ void fill_twiddles(int n, std::vector<double> &tw) {
   }
 }
 
-void launch_poisson(tc_ctx *c) {
+template <int ME>
+static void launch_theta(tc_ctx *c, bool fwd, long long nrows, unsigned blocks, size_t lds) {
+  if (fwd)
+    hipLaunchKernelGGL(k_fft_theta_fwd<ME>, dim3(blocks), dim3(256), lds, c->stream, c->f[TC_DPH],
+                       c->spec, c->plan_t, c->tw_t, nrows, c->nth2);
+  else
+    hipLaunchKernelGGL(k_fft_theta_inv<ME>, dim3(blocks), dim3(256), lds, c->stream, c->spec,
+                       c->f[TC_DPH], c->plan_t, c->tw_t, nrows, c->nth2);
+}
+
+static void theta_pass(tc_ctx *c, bool fwd) {
   const long long nrows = (long long)c->nrm * c->nzm;
   const long long npairs = (nrows + 1) / 2;
-  const FFTPlan &pt = c->plan_t, &pz = c->plan_z;
-  const size_t lds_t = (size_t)pt.lines * (pt.n + 1) * sizeof(double2);
-  const size_t lds_z = (size_t)pz.lines * (pz.n + 1) * sizeof(double2);
-  const unsigned bt = (unsigned)((npairs + pt.lines - 1) / pt.lines);
-  const dim3 gz((unsigned)((c->nth2 + pz.lines - 1) / pz.lines), (unsigned)c->nrm);
-  const long long M = (long long)c->nzm * c->nth2;
+  const FFTPlan &pt = c->plan_t;
+  const size_t lds = (size_t)pt.lines * pt.n * sizeof(double2);
+  const unsigned blocks = (unsigned)((npairs + pt.lines - 1) / pt.lines);
+  switch (pt.me) {
+    case 4: launch_theta<4>(c, fwd, nrows, blocks, lds); break;
+    case 8: launch_theta<8>(c, fwd, nrows, blocks, lds); break;
+    case 16: launch_theta<16>(c, fwd, nrows, blocks, lds); break;
+    default: launch_theta<32>(c, fwd, nrows, blocks, lds); break;
+  }
+}
 
+static void z_pass(tc_ctx *c, double sg) {
+  const FFTPlan &pz = c->plan_z;
+  const size_t lds = (size_t)pz.lines * pz.n * sizeof(double2);
+  const dim3 grid((unsigned)((c->nth2 + pz.lines - 1) / pz.lines), (unsigned)c->nrm);
+  switch (pz.me) {
+    case 4:
+      hipLaunchKernelGGL(k_fft_z<4>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_fft_z<8>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_fft_z<16>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
+      break;
+    default:
+      hipLaunchKernelGGL(k_fft_z<32>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
+      break;
+  }
+}
+
+void launch_poisson(tc_ctx *c) {
+  const long long M = (long long)c->nzm * c->nth2;
   timer_begin(c, "fft_theta_fwd");
-  hipLaunchKernelGGL(k_fft_theta_fwd, dim3(bt), dim3(256), lds_t, c->stream, c->f[TC_DPH], c->spec,
-                     pt, c->tw_t, nrows, c->nth2);
+  theta_pass(c, true);
   timer_end(c);
   timer_begin(c, "fft_z_fwd");
-  hipLaunchKernelGGL(k_fft_z, gz, dim3(256), lds_z, c->stream, c->spec, pz, c->tw_z, c->nth2, -1.0);
+  z_pass(c, -1.0);
   timer_end(c);
   timer_begin(c, "poisson_tridiag");
   hipLaunchKernelGGL(k_poisson_tridiag, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
                      c->gd, c->spec, c->cpw, 1.0 / ((double)c->nthm * (double)c->nzm));
   timer_end(c);
   timer_begin(c, "fft_z_inv");
-  hipLaunchKernelGGL(k_fft_z, gz, dim3(256), lds_z, c->stream, c->spec, pz, c->tw_z, c->nth2, 1.0);
+  z_pass(c, 1.0);
   timer_end(c);
   timer_begin(c, "fft_theta_inv");
-  hipLaunchKernelGGL(k_fft_theta_inv, dim3(bt), dim3(256), lds_t, c->stream, c->spec, c->f[TC_DPH],
-                     pt, c->tw_t, nrows, c->nth2);
+  theta_pass(c, false);
   timer_end(c);
 }
 

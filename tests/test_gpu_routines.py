@@ -25,16 +25,17 @@ def _setup(tc, oracle, shape, seed, masks=False, ns=2, time=0.37, dt=2e-3, ampl=
     orc = oracle.Oracle(p)
     st = random_state(p, seed)
     load_both(dev, orc, st)
+    for s in (dev, orc):
+        s.set_time(time, dt)
+        s.begin_step()
+        s.set_substep(ns)
     if masks:
+        # after begin_step: TimeMarcher's SlipBCs call would reset them
         rng = np.random.default_rng(seed + 7)
         ic = rng.integers(0, 2, (nzm, nthm)).astype(np.int32)
         oc = rng.integers(0, 2, (nzm, nthm)).astype(np.int32)
         dev.set_masks(ic, oc)
         orc.set_masks(ic, oc)
-    for s in (dev, orc):
-        s.set_time(time, dt)
-        s.begin_step()
-        s.set_substep(ns)
     return p, dev, orc
 
 
@@ -111,3 +112,20 @@ def test_reductions(tc, oracle, shape):
     vo, rr = orc.velocity_stats()
     np.testing.assert_allclose(v, vo, rtol=1e-15)
     np.testing.assert_allclose(r, rr, rtol=1e-12)
+
+
+@pytest.mark.parametrize("strips", [(2, 3, 1), (0, 2, 2), (1, 1, 3)])
+def test_slip_bc_masks(tc, oracle, strips):
+    """SlipBCs.F90:31-90 patterns at several times, as the device context
+    computes them at TimeMarcher's step start."""
+    p = small_params(tc, 48, 40, 17)
+    p.update(minstrt=strips[0], minstrz=strips[1], multmin=strips[2])
+    dev = tc.Solver(p)
+    orc = oracle.Oracle(p)
+    for t in (0.0, 0.123, 0.9, 2.5):
+        for s in (dev, orc):
+            s.set_time(t, 1e-3)
+            s.begin_step()
+        a, b = dev.masks(), orc.masks()
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])

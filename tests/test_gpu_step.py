@@ -65,6 +65,27 @@ def test_steps_match_oracle_small(tc, oracle, fused):
     _compare(dev, orc, p["nrm"])
 
 
+@pytest.mark.parametrize("lpt", ["1", "2"])
+@pytest.mark.parametrize("masks", [False, True])
+def test_fused_variants_match_oracle(tc, oracle, monkeypatch, lpt, masks):
+    """Both tilings of the fused Helmholtz kernel (one or two radial lines per
+    thread), with uniform and with mixed slip/no-slip wall masks."""
+    monkeypatch.setenv("TC_HELM_LPT", lpt)
+    p = tc.configs.scaled(tc.configs.small_64(), 96, 40, 29)
+    p.update(sizepert=0.05, omegapert=0.3, virotext=0.2, romeg=0.1)
+    if masks:
+        # free-slip strips on the inner cylinder from SlipBCs
+        p.update(minstrt=2, minstrz=3, multmin=1)
+    dev, orc = _start(tc, oracle, p, 17, 1e-2)
+    for n in range(4):
+        c1, d1 = dev.step(n)
+        c2, d2 = orc.step(n)
+        assert c1 == c2
+    if masks:
+        assert dev.masks()[0].mean() < 1.0
+    _compare(dev, orc, p["nrm"])
+
+
 def test_variable_dt_steps_match_oracle(tc, oracle):
     p = tc.configs.scaled(tc.configs.production_512(), 48, 40, 33)
     p.update(ntst=5)

@@ -93,15 +93,22 @@ def gpu_leg(tc, p, seed, amp, steps, warmup):
             raise RuntimeError(f"step {ntime} ended with error code {code}: {last}")
     dev.synchronize()
     t1 = time.perf_counter()
-    dev.kernel_timing(False)
-    names = list(kernel_bytes(p).keys()) + ["max_cfl", "max_div"]
+    names = list(kernel_bytes(p).keys())
     times = {}
     for n in set(names):
         ms, cnt = dev.kernel_time(n)
         if cnt:
             times[n] = (ms, cnt)
+    # streaming copy of one field, after the timed region: the achievable
+    # bandwidth on this box and the reference for hardware byte counters
+    dev.kernel_time("", reset=True)
+    for _ in range(5):
+        dev.copy_field("rhs", "dph")
+    ms, cnt = dev.kernel_time("copy", reset=True)
+    dev.kernel_timing(False)
+    copy_gbs = 2 * 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"] / (ms / cnt / 1e3) / 1e9
     dev.close()
-    return (t1 - t0), times, last
+    return (t1 - t0), times, last, copy_gbs
 
 
 def cpu_leg(p, seed, amp, max_seconds):
@@ -149,7 +156,7 @@ def main():
     seed, amp = tc.configs.SEEDS[a.workload], tc.configs.AMPLITUDE[a.workload]
     npts = p["nthm"] * p["nzm"] * p["nrm"]
 
-    elapsed, ktimes, last = gpu_leg(tc, p, seed, amp, a.steps, a.warmup)
+    elapsed, ktimes, last, copy_gbs = gpu_leg(tc, p, seed, amp, a.steps, a.warmup)
     ms_step = 1e3 * elapsed / a.steps
     value = npts * a.steps / elapsed / 1e6
 
@@ -190,7 +197,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_ms": round(dom_ms / dom_cnt, 4),
-                     "bytes_per_launch": kb[dom]},
+                     "bytes_per_launch": kb[dom],
+                     "copy_kernel_gbs": round(copy_gbs, 1)},
     }
     if not a.no_cpu:
         n, t, threads = cpu_leg(p, seed, amp, a.cpu_seconds)

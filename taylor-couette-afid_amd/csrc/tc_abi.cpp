@@ -334,9 +334,11 @@ int tc_create(const tc_params *p, tc_ctx **out) {
   {
     // LDS budgets (bytes) of the FFT tiles; TC_FFT_T_LDS / TC_FFT_Z_LDS override
     const char *et = std::getenv("TC_FFT_T_LDS"), *ez = std::getenv("TC_FFT_Z_LDS");
-    const int lt = et ? std::atoi(et) : 16384, lz = ez ? std::atoi(ez) : 32768;
-    if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, lt);
-    if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, lz);
+    const char *ezt = std::getenv("TC_FFT_Z_NT");
+    const int zt = (ezt && std::atoi(ezt) == 256) ? 256 : 512;
+    const int lt = et ? std::atoi(et) : 16384, lz = ez ? std::atoi(ez) : 65536;
+    if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, lt, 256);
+    if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, lz, zt);
   }
   if (!e.empty()) {
     delete c;
@@ -353,6 +355,7 @@ int tc_create(const tc_params *p, tc_ctx **out) {
   c->nrm = p->nrm;
   c->nr = p->nrm + 1;
   c->nth2 = p->nthm / 2 + 1;
+  c->sstr = (c->nth2 + 7) / 8 * 8;  // spectral rows start on 128-byte lines
   c->P = (long long)c->nzm * c->nthm;
   c->N = (long long)c->nr * c->P;
   int rc = 0;
@@ -378,7 +381,7 @@ int tc_create(const tc_params *p, tc_ctx **out) {
     TRY(dev_alloc(c, (void **)&c->f[q], sizeof(double) * (size_t)c->N));
     HTRY(hipMemsetAsync(c->f[q], 0, sizeof(double) * (size_t)c->N, c->stream));
   }
-  const size_t M = (size_t)c->nzm * c->nth2;
+  const size_t M = (size_t)c->nzm * c->sstr;
   TRY(dev_alloc(c, (void **)&c->spec, sizeof(double2) * M * c->nrm));
   TRY(dev_alloc(c, (void **)&c->cpw, sizeof(double) * M * c->nrm));
   // grid vectors
@@ -421,6 +424,7 @@ int tc_create(const tc_params *p, tc_ctx **out) {
     c->gd.nrm = c->nrm;
     c->gd.nr = c->nr;
     c->gd.nth2 = c->nth2;
+    c->gd.sstr = c->sstr;
     c->gd.P = c->P;
   }
   // masks (SlipBCs at time 0)

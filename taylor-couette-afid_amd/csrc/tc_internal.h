@@ -21,7 +21,7 @@ struct GridDev {
   const double *apph, *amph, *acph, *ak1, *ak3;
   double dxt, dxz, dxtq, dxzq, ren, romeg;
   double ap1j_fs2, ac1j_fs1, ac1j_fs2, am1j_fs1, ap3j_fs2, ac3j_fs1, ac3j_fs2, am3j_fs1;
-  int nthm, nzm, nrm, nr, nth2;
+  int nthm, nzm, nrm, nr, nth2, sstr;  // sstr: spectral row stride >= nth2
   long long P;  // points per r-plane = nzm*nthm
 };
 
@@ -31,7 +31,8 @@ struct FFTPlan {
   int nst = 0;
   int radix[24] = {0};
   int lines = 1;  // FFT lines per 256-thread block
-  int me = 8;     // points per thread per pass (n <= me * 256/lines)
+  int me = 8;     // points per thread per pass (n <= me * nt/lines)
+  int nt = 256;   // threads per block
 };
 
 // Helmholtz LU tables: per (substep slot, component, mask variant) a block of
@@ -48,13 +49,13 @@ struct Timer {
 struct tc_ctx {
   tc_params p;
   tc::Grid g;
-  int nthm = 0, nzm = 0, nrm = 0, nr = 0, nth2 = 0;
+  int nthm = 0, nzm = 0, nrm = 0, nr = 0, nth2 = 0, sstr = 0;
   long long P = 0, N = 0;  // plane size, field size (nr planes)
   hipStream_t stream = nullptr;
   int device = 0;
   double *f[TC_NFIELDS] = {nullptr};
-  double2 *spec = nullptr;   // nrm x nzm x nth2 complex, [r][kz][ktheta]
-  double *cpw = nullptr;     // nrm x nzm x nth2 real: Thomas multipliers
+  double2 *spec = nullptr;   // nrm x nzm x sstr complex, [r][kz][ktheta]
+  double *cpw = nullptr;     // nrm x nzm x sstr real: Thomas multipliers
   double *gvbuf = nullptr;   // device grid vectors
   tc::GridDev gd;
   unsigned char *mask = nullptr;  // device [2][nzm*nthm] (icyl, ocyl), [j][i]
@@ -117,7 +118,7 @@ void launch_diagnostics(tc_ctx *c);
 void launch_copy(tc_ctx *c, int dst, int src);
 
 // Poisson solve (tc_poisson.hip)
-std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes);
+std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes, int nt);
 void fill_twiddles(int n, std::vector<double> &tw);
 void launch_poisson(tc_ctx *c);
 

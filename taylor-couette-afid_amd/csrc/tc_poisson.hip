@@ -168,7 +168,7 @@ template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict__ in,
                                                        double2 *__restrict__ spec, FFTPlan pl,
                                                        const double2 *__restrict__ tw,
-                                                       long long nrows, int nth2) {
+                                                       long long nrows, int nth2, int sstr) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = 256 / L;
   const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
@@ -186,8 +186,8 @@ __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict_
     // A = (Z + conj(Zc))/2, B = (Z - conj(Zc))/(2i)
     const double2 A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
     const double2 B = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
-    if (v0) spec[r0 * nth2 + k] = A;
-    if (v1) spec[r1 * nth2 + k] = B;
+    if (v0) spec[r0 * sstr + k] = A;
+    if (v1) spec[r1 * sstr + k] = B;
   }
 }
 
@@ -197,7 +197,7 @@ template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict__ spec,
                                                        double *__restrict__ out, FFTPlan pl,
                                                        const double2 *__restrict__ tw,
-                                                       long long nrows, int nth2) {
+                                                       long long nrows, int nth2, int sstr) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = 256 / L;
   const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
@@ -208,8 +208,8 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
   for (int k = t; k < n; k += T) {
     const bool lo = (k < nth2);
     const int kk = lo ? k : n - k;
-    double2 A = v0 ? spec[r0 * nth2 + kk] : make_double2(0.0, 0.0);
-    double2 B = v1 ? spec[r1 * nth2 + kk] : make_double2(0.0, 0.0);
+    double2 A = v0 ? spec[r0 * sstr + kk] : make_double2(0.0, 0.0);
+    double2 B = v1 ? spec[r1 * sstr + kk] : make_double2(0.0, 0.0);
     if (!lo) {
       A.y = -A.y;
       B.y = -B.y;
@@ -229,27 +229,38 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
   }
 }
 
+// Workgroups are dispatched round-robin over the XCDs; hand each XCD a
+// contiguous run of tiles so tiles sharing cache lines share one L2.
+DEVI int xcd_tile(int b, int nb) {
+  constexpr int NX = 8;
+  const int q = nb / NX, r = nb - q * NX;
+  const int x = b % NX, local = b / NX;
+  return x * q + (x < r ? x : r) + local;
+}
+
 // c2c along z for a tile of pl.lines ktheta columns of one r-plane.  The LDS
 // tile keeps the global row-major order [z][column]; lane = column fastest,
-// so global and LDS accesses are both contiguous across lanes.
-template <int ME>
-__global__ __launch_bounds__(256) void k_fft_z(double2 *__restrict__ spec, FFTPlan pl,
-                                               const double2 *__restrict__ tw, int nth2,
-                                               double sg) {
+// so global and LDS accesses are both contiguous across lanes.  Spectral rows
+// are sstr complex apart (a multiple of 8, i.e. of 128 bytes).
+template <int ME, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPlan pl,
+                                              const double2 *__restrict__ tw, int nth2, int sstr,
+                                              int ntile, double sg) {
   extern __shared__ double2 lds[];
-  const int n = pl.n, L = pl.lines, T = 256 / L;
+  const int n = pl.n, L = pl.lines, T = NT / L;
   const int c = threadIdx.x % L, t = threadIdx.x / L;
-  const int c0 = blockIdx.x * L;
-  double2 *plane = spec + (long long)blockIdx.y * n * nth2;
-  for (int e = threadIdx.x; e < L * n; e += 256) {
+  const int id = xcd_tile(blockIdx.x, gridDim.x);
+  const int c0 = (id % ntile) * L;
+  double2 *plane = spec + (long long)(id / ntile) * n * sstr;
+  for (int e = threadIdx.x; e < L * n; e += NT) {
     const int z = e / L, cc = e - z * L;
-    lds[e] = (c0 + cc < nth2) ? plane[(long long)z * nth2 + c0 + cc] : make_double2(0.0, 0.0);
+    lds[e] = (c0 + cc < nth2) ? plane[(long long)z * sstr + c0 + cc] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   fft_line<ME>(lds + c, L, pl, t, T, tw, sg);
-  for (int e = threadIdx.x; e < L * n; e += 256) {
+  for (int e = threadIdx.x; e < L * n; e += NT) {
     const int z = e / L, cc = e - z * L;
-    if (c0 + cc < nth2) plane[(long long)z * nth2 + c0 + cc] = lds[e];
+    if (c0 + cc < nth2) plane[(long long)z * sstr + c0 + cc] = lds[e];
   }
 }
 
@@ -262,10 +273,11 @@ __global__ __launch_bounds__(256) void k_poisson_tridiag(GridDev g, double2 *__r
                                                          double *__restrict__ cpw,
                                                          double inv_norm) {
   constexpr int CH = 8;  // rows loaded ahead of the recurrence
-  const long long M = (long long)g.nzm * g.nth2;
+  const long long M = (long long)g.nzm * g.sstr;
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
-  const int kz = (int)(m / g.nth2), kt = (int)(m - (long long)kz * g.nth2);
+  const int kz = (int)(m / g.sstr), kt = (int)(m - (long long)kz * g.sstr);
+  if (kt >= g.nth2) return;  // row padding
   const double sz = g.ak3[kz + 1], st = g.ak1[kt + 1];
   const int nn = (m == 0) ? g.nrm - 1 : g.nrm;
   // forward elimination
@@ -324,9 +336,10 @@ __global__ __launch_bounds__(256) void k_poisson_tridiag(GridDev g, double2 *__r
   if (m == 0) spec[(long long)(g.nrm - 1) * M] = make_double2(0.0, 0.0);
 }
 
-std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes) {
+std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes, int nt) {
   pl = FFTPlan();
   pl.n = n;
+  pl.nt = nt;
   int m = n;
   while (m % 8 == 0 && m > 1) { pl.radix[pl.nst++] = 8; m /= 8; }
   while (m % 4 == 0 && m > 1) { pl.radix[pl.nst++] = 4; m /= 4; }
@@ -338,8 +351,9 @@ std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes) {
   // 64); then the points per thread per pass, ME, from {4, 8, 16, 32}
   int L = 64;
   while (L > 1 && (long long)L * n * 16 > lds_bytes) L /= 2;
+  while (L > 1 && nt / L < 2) L /= 2;
   for (;;) {
-    const int T = 256 / L;
+    const int T = nt / L;
     const int need = (n + T - 1) / T;
     int me = 4;
     while (me < need) me *= 2;
@@ -370,10 +384,10 @@ template <int ME>
 static void launch_theta(tc_ctx *c, bool fwd, long long nrows, unsigned blocks, size_t lds) {
   if (fwd)
     hipLaunchKernelGGL(k_fft_theta_fwd<ME>, dim3(blocks), dim3(256), lds, c->stream, c->f[TC_DPH],
-                       c->spec, c->plan_t, c->tw_t, nrows, c->nth2);
+                       c->spec, c->plan_t, c->tw_t, nrows, c->nth2, c->sstr);
   else
     hipLaunchKernelGGL(k_fft_theta_inv<ME>, dim3(blocks), dim3(256), lds, c->stream, c->spec,
-                       c->f[TC_DPH], c->plan_t, c->tw_t, nrows, c->nth2);
+                       c->f[TC_DPH], c->plan_t, c->tw_t, nrows, c->nth2, c->sstr);
 }
 
 static void theta_pass(tc_ctx *c, bool fwd) {
@@ -390,28 +404,33 @@ static void theta_pass(tc_ctx *c, bool fwd) {
   }
 }
 
-static void z_pass(tc_ctx *c, double sg) {
+template <int NT>
+static void z_pass_nt(tc_ctx *c, double sg) {
   const FFTPlan &pz = c->plan_z;
   const size_t lds = (size_t)pz.lines * pz.n * sizeof(double2);
-  const dim3 grid((unsigned)((c->nth2 + pz.lines - 1) / pz.lines), (unsigned)c->nrm);
+  const int ntile = (c->nth2 + pz.lines - 1) / pz.lines;
+  const dim3 grid((unsigned)(ntile * c->nrm));
+#define TC_ZLAUNCH(ME)                                                                          \
+  hipLaunchKernelGGL((k_fft_z<ME, NT>), grid, dim3(NT), lds, c->stream, c->spec, pz, c->tw_z, \
+                     c->nth2, c->sstr, ntile, sg)
   switch (pz.me) {
-    case 4:
-      hipLaunchKernelGGL(k_fft_z<4>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
-      break;
-    case 8:
-      hipLaunchKernelGGL(k_fft_z<8>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
-      break;
-    case 16:
-      hipLaunchKernelGGL(k_fft_z<16>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
-      break;
-    default:
-      hipLaunchKernelGGL(k_fft_z<32>, grid, dim3(256), lds, c->stream, c->spec, pz, c->tw_z, c->nth2, sg);
-      break;
+    case 4: TC_ZLAUNCH(4); break;
+    case 8: TC_ZLAUNCH(8); break;
+    case 16: TC_ZLAUNCH(16); break;
+    default: TC_ZLAUNCH(32); break;
   }
+#undef TC_ZLAUNCH
+}
+
+static void z_pass(tc_ctx *c, double sg) {
+  if (c->plan_z.nt == 512)
+    z_pass_nt<512>(c, sg);
+  else
+    z_pass_nt<256>(c, sg);
 }
 
 void launch_poisson(tc_ctx *c) {
-  const long long M = (long long)c->nzm * c->nth2;
+  const long long M = (long long)c->nzm * c->sstr;
   timer_begin(c, "fft_theta_fwd");
   theta_pass(c, true);
   timer_end(c);

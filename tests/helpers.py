@@ -63,3 +63,32 @@ def pressure_modulo_gauge(pr, ref):
     singular (0,0) Poisson mode leaves undetermined in the reference."""
     d = pr - ref
     return d - d.mean(axis=(1, 2), keepdims=True)
+
+
+def write_bou(path, p):
+    """A bou.in with the eleven header/value line pairs of ReadInputFile.F90:17-37."""
+    g = lambda k: p.get(k, 0)
+    lines = [
+        "NTM NZM NRM NSST IDTV NREAD",
+        f"{g('nthm')} {g('nzm')} {g('nrm')} {g('nsst')} {g('idtv')} {g('nread')}",
+        "NTST TOUT TMAX IRESET WALLTIMEMAX",
+        f"{g('ntst')} {g('tout')!r} {g('tmax')!r} {g('ireset')} {g('walltimemax')!r}",
+        "ALX3D RINT REXT ISTR STR3 LAMB",
+        f"{g('alx3')!r} {g('rint')!r} {g('rext')!r} {g('istr')} {g('str')!r} {g('lamb')!r}",
+        "VIROTINT VIROTEXT AMPL_PERT FREQ_PERT",
+        f"{g('virotint')!r} {g('virotext')!r} {g('sizepert')!r} {g('omegapert')!r}",
+        "1/NU ROTP RESID CFLMAX",
+        f"{g('ren')!r} {g('romeg')!r} {g('resid')!r} {g('limitcfl')!r}",
+        "TSTA STAREAD",
+        f"{g('tsta')!r} {g('starea')}",
+        "DTMIN DTMAX VMAX CFLLIM",
+        f"{g('dtmin')!r} {g('dtmax')!r} {g('limitvel')!r} {g('cfllim')!r}",
+        "NOUT_RCUT OUT_THCUT OUT_STWAVE TFRAME2D",
+        f"{g('nslabr')} {g('dpthcut')} {g('dpstave')} {g('tframe2d')!r}",
+        "OUT_3DFIELD TFRAME3D",
+        f"{g('dpfullf')} {g('tframe3d')!r}",
+        "MINSTRT MINSTRZ MULTMIN",
+        f"{g('minstrt')} {g('minstrz')} {g('multmin')}",
+    ]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

@@ -1,0 +1,101 @@
+"""The Fortran host (tc_host: bou.in, main loop, AFiD routine names over the
+C-ABI) on the device, routine by routine and fused, against the oracle run
+of the same bou.in: torque at every step within 1e-9, final fields within
+1e-10 relative L2."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2, write_bou
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "taylor-couette-afid_amd", "tc_host")
+
+
+def _params(tc):
+    p = tc.configs.scaled(tc.configs.small_64(), 32, 24, 33)
+    p.update(ntst=5, tout=1e-9, sizepert=0.1, omegapert=0.5, idtv=1, dtmin=1e-3, dtmax=5e-3,
+             limitcfl=0.5, virotext=-0.2)
+    return p
+
+
+def _run_host(tmp, p, fused):
+    write_bou(tmp / "bou.in", p)
+    cmd = [HOST, "--ic", "perturbed", "77", "0.01"] + (["--fused"] if fused else [])
+    r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "exit code 555" in r.stdout, r.stdout
+    tq = np.loadtxt(tmp / "torquenu_full.out", ndmin=2)
+    shape = (p["nrm"] + 1, p["nzm"], p["nthm"])
+    fields = {n: np.fromfile(tmp / f"continua_{n}.bin").reshape(shape, order="F")
+              for n in ("qt", "qr", "qz", "pr")}
+    return tq, fields
+
+
+def _oracle_run(tc, oracle, p):
+    dev = tc.Solver(p)
+    qt, qr, qz = tc.init_perturbed_couette(p, 77, 0.01)
+    dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+    dev.project()
+    o = oracle.Oracle(p)
+    for n in ("qt", "qr", "qz"):
+        o.set(n, dev.get(n))
+    dev.close()
+    tq = []
+    for n in range(p["ntst"] + 1):
+        code, d = o.step(n)
+        tq.append((d["time"], d["torque_in"], d["torque_out"]))
+        if code:
+            break
+    assert code == 555
+    return np.array(tq), {n: o.get(n) for n in ("qt", "qr", "qz", "pr")}
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_fortran_host_matches_oracle(tc, oracle, tmp_path, fused):
+    if not os.path.exists(HOST):
+        pytest.fail("tc_host is not built")
+    p = _params(tc)
+    tq, f = _run_host(tmp_path, p, fused)
+    tq_o, f_o = _oracle_run(tc, oracle, p)
+    assert tq.shape == tq_o.shape
+    np.testing.assert_allclose(tq[:, 0], tq_o[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(tq[:, 1:], tq_o[:, 1:], rtol=1e-9)
+    nrm = p["nrm"]
+    for n in ("qt", "qz"):
+        assert rel_l2(f[n][:nrm], f_o[n][:nrm]) < 1e-10, n
+    assert rel_l2(f["qr"], f_o["qr"]) < 1e-10
+    d = f["pr"][:nrm] - f_o["pr"][:nrm]
+    d -= d.mean(axis=(1, 2), keepdims=True)
+    ref = f_o["pr"][:nrm] - f_o["pr"][:nrm].mean(axis=(1, 2), keepdims=True)
+    assert np.linalg.norm(d) / np.linalg.norm(ref) < 1e-10
+
+
+def test_error_codes_match_oracle(tc, oracle):
+    """main.F90 exit codes forced from bou.in values, device against oracle."""
+    base = tc.configs.scaled(tc.configs.small_64(), 8, 8, 9)
+    base.update(ren=100.0, tout=1e30, ntst=50, tmax=1e30)
+    cases = [dict(ntst=2), dict(resid=1e-300), dict(tmax=1e-6), dict(limitvel=1e-9),
+             dict(idtv=0, dtmax=1e-3, limitcfl=1e-9),
+             dict(idtv=1, dtmin=2e-3, dtmax=1e-2, limitcfl=1e-9)]
+    for case in cases:
+        p = dict(base, **case)
+        codes = []
+        for side in ("dev", "orc"):
+            s = tc.Solver(p) if side == "dev" else oracle.Oracle(p)
+            rng = np.random.default_rng(1)
+            q = np.zeros((p["nrm"] + 1, p["nzm"], p["nthm"]), order="F")
+            q[1:p["nrm"]] = 1e-2 * rng.uniform(-1, 1, (p["nrm"] - 1, p["nzm"], p["nthm"]))
+            s.set("qr", q)
+            (s.project() if side == "dev" else s.call("project"))
+            seq = []
+            for k in range(5):
+                c, _ = s.step(k)
+                seq.append(c)
+                if c:
+                    break
+            codes.append(seq)
+        assert codes[0] == codes[1], (case, codes)

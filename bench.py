@@ -25,6 +25,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Mpt\u00b7RK3-steps/s FP64 at 512\u00b2\u00d7385 + %HBM roofline; sweep jobs/h, p99"
+PROFILES = os.path.join(ROOT, "profiles")
+
+# rocprofv3 commands whose output --summarize-profile turns into the committed
+# summaries (run from /tmp with TMPDIR=/tmp; OUT is the output folder)
+PROFILE_COMMANDS = {
+    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o bench "
+             "-- python3 bench.py --steps 5 --warmup 1 --no-cpu",
+    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE --output-format csv --kernel-include-regex "
+                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_divergence|k_diagnostics\" "
+                  "-d OUT/FETCH_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 --no-cpu",
+    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE --output-format csv --kernel-include-regex "
+                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_divergence|k_diagnostics\" "
+                  "-d OUT/WRITE_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 --no-cpu",
+}
 
 
 def kernel_bytes(p):
@@ -138,6 +153,104 @@ def cpu_leg(p, seed, amp, max_seconds):
     return n, t_steps, threads
 
 
+def _kernel_short(name):
+    """'void tc::k_helm_fwd<true, 2>(tc::GridDev, ...)' -> 'k_helm_fwd<true, 2>'."""
+    name = name.replace("void ", "").split("(")[0]
+    return name.split("::", 1)[1] if "::" in name else name
+
+
+def _timer_of(kernel):
+    """Kernel timer name (the kernel_bytes key) of a device kernel name."""
+    base = kernel.split("<")[0]
+    return base[2:] if base.startswith("k_") else base
+
+
+def summarize_profile(out_dir, workload, p):
+    """Turn the CSV output of PROFILE_COMMANDS into the two text summaries
+    under profiles/: per-kernel durations, and HBM bytes per dispatch from
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same
+    counters read for the streaming copy of one field of known size."""
+    import csv
+    import collections
+
+    kb = kernel_bytes(p)
+    N = p["nthm"] * p["nzm"] * p["nrm"]
+    R = 8 * N
+    copy_bytes = 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"]
+
+    rows = list(csv.DictReader(open(os.path.join(out_dir, "trace", "bench_kernel_stats.csv"))))
+    lines = [f"# Kernel durations, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']}, theta x z x r)", "",
+             "Command (from /tmp, TMPDIR=/tmp, OUT = an output folder, repository root as",
+             "the working directory of bench.py):", "", "    " + PROFILE_COMMANDS["trace"], "",
+             "5 timed steps + 1 warm-up step + 5 copies of one field.  Algorithmic GB/s =",
+             "the kernel's distinct input and output arrays (bench.py kernel_bytes) / average",
+             "duration.", "",
+             "| kernel | calls | average us | min us | max us | share % | algorithmic GB | GB/s |",
+             "|---|---|---|---|---|---|---|---|"]
+    for r in rows:
+        k = _kernel_short(r["Name"])
+        t = _timer_of(k)
+        avg = float(r["AverageNs"]) / 1e3
+        b = kb.get(t) or kb.get(t + "_fwd")
+        if t == "copy":
+            b = 2 * copy_bytes
+        gb = f"{b / 1e9:.3f}" if b else "-"
+        gbs = f"{b / (avg * 1e-6) / 1e9:.0f}" if b else "-"
+        lines.append(f"| {k} | {r['Calls']} | {avg:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} | {gb} | {gbs} |")
+    stats_path = os.path.join(PROFILES, f"all_kernels_{workload}_stats.md")
+    open(stats_path, "w").write("\n".join(lines) + "\n")
+
+    def load(counter):
+        d = collections.defaultdict(list)
+        path = os.path.join(out_dir, counter, "pmc_counter_collection.csv")
+        for r in csv.DictReader(open(path)):
+            d[_kernel_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in d.items()}, {k: len(v) for k, v in d.items()}
+
+    fe, cnt = load("FETCH_SIZE")
+    wr, _ = load("WRITE_SIZE")
+    kf = copy_bytes / (fe["k_copy"] * 1024)
+    kw = copy_bytes / (wr["k_copy"] * 1024)
+    lines = [f"# HBM traffic per dispatch, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']})", "",
+             "Commands (from /tmp, TMPDIR=/tmp, OUT = an output folder), one counter per run:", "",
+             "    " + PROFILE_COMMANDS["FETCH_SIZE"], "    " + PROFILE_COMMANDS["WRITE_SIZE"], "",
+             f"Calibration: the streaming copy of one field moves {copy_bytes} B each way; its",
+             f"FETCH_SIZE reads {fe['k_copy']:.0f} KiB and its WRITE_SIZE {wr['k_copy']:.0f} KiB per "
+             f"dispatch, so", f"read bytes = FETCH_SIZE x 1024 x {kf:.4f} and write bytes = WRITE_SIZE "
+             f"x 1024 x {kw:.4f}.", f"One field pass (8 B x {N} cells) = {R / 1e9:.4f} GB.", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per dispatch | "
+             "field passes | algorithmic passes |", "|---|---|---|---|---|---|---|"]
+    for k in sorted(fe, key=lambda k: -fe[k] - wr.get(k, 0)):
+        t = _timer_of(k)
+        b = kb.get(t) or kb.get(t + "_fwd")
+        if t == "copy":
+            b = 2 * copy_bytes
+        hbm = fe[k] * 1024 * kf + wr.get(k, 0.0) * 1024 * kw
+        lines.append(f"| {k} | {cnt[k]} | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {hbm:.0f} | "
+                     f"{hbm / R:.2f} | {(b / R) if b else float('nan'):.2f} |")
+    traffic_path = os.path.join(PROFILES, f"all_kernels_{workload}_hbm_traffic.md")
+    open(traffic_path, "w").write("\n".join(lines) + "\n")
+    print(stats_path)
+    print(traffic_path)
+
+
+def traffic_of(workload, timer):
+    """HBM bytes per dispatch of a kernel from the committed counter summary,
+    or None when there is none for this workload."""
+    path = os.path.join(PROFILES, f"all_kernels_{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")[1:-1]]
+        if len(cells) == 7 and cells[0].startswith("k_") and _timer_of(cells[0]) in (timer, timer.rsplit("_", 1)[0]):
+            try:
+                return int(float(cells[4]))
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +259,13 @@ def main():
     ap.add_argument("--workload", default="production_512")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--summarize-profile", metavar="OUT",
+                    help="write the profiles/ summaries from rocprofv3 CSV output in OUT and exit")
     a = ap.parse_args()
+    if a.summarize_profile:
+        tc = importlib.import_module("taylor-couette-afid_amd")
+        summarize_profile(a.summarize_profile, a.workload, tc.configs.WORKLOADS[a.workload]())
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the multi-GPU sharding is a design, not a run)"}))
@@ -177,9 +296,9 @@ def main():
           f"{step_gb / (ms_step / 1e3):.0f} GB/s equivalent; last {last}", file=sys.stderr)
 
     out = {
-        "metric": "Mpt*RK3-steps/s FP64 (512^2x385) + %HBM roofline",
+        "metric": METRIC,
         "value": round(value, 3),
-        "unit": "Mpt*steps/s",
+        "unit": "Mpt\u00b7RK3-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
@@ -195,14 +314,15 @@ def main():
                    "grid_points": npts},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_of(a.workload, dom),
                      "avg_launch_ms": round(dom_ms / dom_cnt, 4),
                      "bytes_per_launch": kb[dom],
                      "copy_kernel_gbs": round(copy_gbs, 1)},
     }
     if not a.no_cpu:
         n, t, threads = cpu_leg(p, seed, amp, a.cpu_seconds)
-        out["cpu_baseline"] = {"value": round(npts * n / t / 1e6, 4), "unit": "Mpt*steps/s",
+        out["cpu_baseline"] = {"value": round(npts * n / t / 1e6, 4), "unit": "Mpt\u00b7RK3-steps/s",
                                "cores": threads, "kind": "port",
                                "sample": f"{n} RK3 step(s) of the same {p['nthm']}x{p['nzm']}x"
                                          f"{p['nrm']} workload from the same start, oracle "

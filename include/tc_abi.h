@@ -106,19 +106,19 @@ int tc_set_substep(tc_ctx *ctx, int ns);
 
 /* ---------------- the hot-path routines ---------------- */
 
-int tc_explicit_terms_qt(tc_ctx *ctx);       /* ExplicitTermsQT.F90:78-133      */
-int tc_explicit_terms_qr(tc_ctx *ctx);       /* ExplicitTermsQR.F90:214-353     */
-int tc_explicit_terms_qz(tc_ctx *ctx);       /* ExplicitTermsQZ.F90:366-465     */
+int tc_explicit_terms_qt(tc_ctx *ctx);       /* ExplicitTermsQT.F90:11-133      */
+int tc_explicit_terms_qr(tc_ctx *ctx);       /* ExplicitTermsQR.F90:13-153      */
+int tc_explicit_terms_qz(tc_ctx *ctx);       /* ExplicitTermsQZ.F90:11-110      */
 int tc_implicit_and_update_qt(tc_ctx *ctx);  /* ImplicitAndUpdateQT.F90:14-124 +
-                                                SolveImpEqnUpdate_QT.F90:136-208 */
-int tc_implicit_and_update_qr(tc_ctx *ctx);  /* ImplicitAndUpdateQR.F90:223-283 +
-                                                SolveImpEqnUpdate_QR.F90:295-355 */
+                                                SolveImpEqnUpdate_QT.F90:12-84 */
+int tc_implicit_and_update_qr(tc_ctx *ctx);  /* ImplicitAndUpdateQR.F90:15-74  +
+                                                SolveImpEqnUpdate_QR.F90:12-72 */
 int tc_implicit_and_update_qz(tc_ctx *ctx);  /* ImplicitAndUpdateQZ.F90:14-123 +
-                                                SolveImpEqnUpdate_QZ.F90:134-197 */
+                                                SolveImpEqnUpdate_QZ.F90:11-74 */
 int tc_calc_local_divergence(tc_ctx *ctx);   /* CalcLocalDivergence.F90:11-44   */
-int tc_solve_pressure_correction(tc_ctx *ctx);/* SolvePressureCorrection.F90:56-217 */
-int tc_correct_velocity(tc_ctx *ctx);        /* CorrectVelocity.F90:229-267     */
-int tc_correct_pressure(tc_ctx *ctx);        /* CorrectPressure.F90:279-320     */
+int tc_solve_pressure_correction(tc_ctx *ctx);/* SolvePressureCorrection.F90:12-173 */
+int tc_correct_velocity(tc_ctx *ctx);        /* CorrectVelocity.F90:12-50       */
+int tc_correct_pressure(tc_ctx *ctx);        /* CorrectPressure.F90:11-52       */
 
 /* TimeMarcher.F90:12-57: one RK3 (or AB2) step at the current dt,
  * including tc_begin_step.  Uses fused kernels unless tc_set_fused(ctx,0). */
@@ -128,8 +128,8 @@ int tc_set_fused(tc_ctx *ctx, int fused);
 /* ---------------- reductions (synchronise) ---------------- */
 
 int tc_calc_max_cfl(tc_ctx *ctx, double *cflm);        /* CalcMaxCFL.F90:12-49     */
-int tc_check_divergence(tc_ctx *ctx, double *qmax);    /* CheckDivergence.F90:60-95 */
-int tc_calc_torque(tc_ctx *ctx, double *inner, double *outer); /* CalcTorque.F90:107-157 */
+int tc_check_divergence(tc_ctx *ctx, double *qmax);    /* CheckDivergence.F90:11-46 */
+int tc_calc_torque(tc_ctx *ctx, double *inner, double *outer); /* CalcTorque.F90:12-62 */
 int tc_calc_velocity_stats(tc_ctx *ctx, double *vmax3, double *rms3); /* CalcVelocityStats.F90:11-68 */
 
 /* ---------------- driver helpers ---------------- */
@@ -147,7 +147,7 @@ int tc_step(tc_ctx *ctx, int ntime, double walltime, double *out9);
 int tc_project(tc_ctx *ctx);
 
 /* Seeded starting field in Fortran layout (nr,nzm,nthm): laminar Couette
- * qt = A rm + B/rm (CreateInitialConditions.F90:180-189) plus i.i.d.
+ * qt = A rm + B/rm (CreateInitialConditions.F90:21-30) plus i.i.d.
  * U(-amp,amp) on qt (k=1..nrm), qr (k=2..nrm) and qz (k=1..nrm) from a
  * splitmix64 stream; the k=nr plane and the qr walls are zero. */
 int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed,

@@ -22,7 +22,7 @@
  *     there, main.F90:158-174, and divides by zero);
  *   - the singular (kz,ktheta)=(0,0) Poisson mode is solved on rows 1..nrm-1
  *     with phi(nrm)=0 instead of LAPACK's roundoff-defined last pivot
- *     (SolvePressureCorrection.F90:183-186).
+ *     (SolvePressureCorrection.F90:139-142).
  *
  * Third-party arithmetic restated here: LAPACK dgttrf/dgttrs/zgttrf/zgttrs
  * (reference LAPACK algorithm, partial pivoting) and the unnormalised DFT
@@ -611,7 +611,7 @@ void or_set_substep(or_state *s, int ns) {
 }
 
 /* ------------------------------------------------------------------ */
-/* ExplicitTermsQT.F90:89-196                                         */
+/* ExplicitTermsQT.F90:22-125                                          */
 /* ------------------------------------------------------------------ */
 void or_explicit_terms_qt(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -652,7 +652,7 @@ void or_explicit_terms_qt(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* ExplicitTermsQR.F90:226-350 (kc=2..nrm; dph(1) untouched)           */
+/* ExplicitTermsQR.F90:25-146  (kc=2..nrm; dph(1) untouched)           */
 /* ------------------------------------------------------------------ */
 void or_explicit_terms_qr(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -693,7 +693,7 @@ void or_explicit_terms_qr(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* ExplicitTermsQZ.F90:378-461                                        */
+/* ExplicitTermsQZ.F90:23-102                                         */
 /* ------------------------------------------------------------------ */
 void or_explicit_terms_qz(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -906,7 +906,7 @@ void or_zgttrs(int n, const double *dl, const double *d, const double *du, const
 }
 
 /* ------------------------------------------------------------------ */
-/* ImplicitAndUpdateQT.F90:26-121 + SolveImpEqnUpdate_QT.F90:146-204    */
+/* ImplicitAndUpdateQT.F90:26-121 + SolveImpEqnUpdate_QT.F90:22-78      */
 /* ------------------------------------------------------------------ */
 static void solve_imp_eqn_update_qt(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -1002,7 +1002,7 @@ void or_implicit_and_update_qt(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* ImplicitAndUpdateQR.F90:36-79 + SolveImpEqnUpdate_QR.F90:308-351     */
+/* ImplicitAndUpdateQR.F90:28-71 + SolveImpEqnUpdate_QR.F90:25-65       */
 /* ------------------------------------------------------------------ */
 void or_implicit_and_update_qr(or_state *s) {
   const int nrm = s->nrm, nr = s->nr, nzm = s->nzm, nthm = s->nthm;
@@ -1061,7 +1061,7 @@ void or_implicit_and_update_qr(or_state *s) {
     free(fkl);
   }
   free(amkl); free(apkl); free(ackl); free(amkT); free(apkT); free(appk); free(ackT); free(ipkv);
-  /* ImplicitAndUpdateQR.F90:78-79: whole wall slabs, halos included */
+  /* ImplicitAndUpdateQR.F90:70-71: whole wall slabs, halos included */
   for (int i = 0; i <= nthm + 1; i++)
     for (int j = 0; j <= nzm + 1; j++) {
       QR(1, j, i) = 0.0;
@@ -1070,7 +1070,7 @@ void or_implicit_and_update_qr(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* ImplicitAndUpdateQZ.F90:24-119 + SolveImpEqnUpdate_QZ.F90:144-193    */
+/* ImplicitAndUpdateQZ.F90:24-119 + SolveImpEqnUpdate_QZ.F90:21-68      */
 /* ------------------------------------------------------------------ */
 static void solve_imp_eqn_update_qz(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -1187,7 +1187,7 @@ void or_calc_local_divergence(or_state *s) {
 
 /* ------------------------------------------------------------------ */
 /* FFT: unnormalised DFT X[k] = sum_n x[n] exp(sign*2*pi*i*n*k/N)      */
-/* (the transform FFTW computes, SolvePressureCorrection.F90:105-139)  */
+/* (the transform FFTW computes, SolvePressureCorrection.F90:106-113)  */
 /* recursive mixed-radix decimation in time over the prime factors of  */
 /* N, twiddles from a per-length table exp(-2 pi i m/N)                */
 /* ------------------------------------------------------------------ */
@@ -1322,7 +1322,7 @@ void or_fft_c2r(int n, const double *in, double *out) {
 }
 
 /* ------------------------------------------------------------------ */
-/* SolvePressureCorrection.F90:93-208                                 */
+/* SolvePressureCorrection.F90:100-163                                */
 /* layout of the spectrum: (k=1..nrm, kz=1..nzmh, ktheta=1..nthm)     */
 /* ------------------------------------------------------------------ */
 void or_solve_pressure_correction(or_state *s) {
@@ -1461,7 +1461,7 @@ void or_solve_pressure_correction(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* CorrectVelocity.F90:245-264                                        */
+/* CorrectVelocity.F90:28-47                                          */
 /* ------------------------------------------------------------------ */
 void or_correct_velocity(or_state *s) {
   const int nrm = s->nrm, nr = s->nr, nzm = s->nzm, nthm = s->nthm;
@@ -1488,7 +1488,7 @@ void or_correct_velocity(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* CorrectPressure.F90:291-318 (amm = apph, be = beta)                  */
+/* CorrectPressure.F90:23-46   (amm = apph, be = beta)                  */
 /* ------------------------------------------------------------------ */
 void or_correct_pressure(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -1578,7 +1578,7 @@ double or_calc_max_cfl(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* CheckDivergence.F90:70-92                                          */
+/* CheckDivergence.F90:21-43                                          */
 /* ------------------------------------------------------------------ */
 double or_check_divergence(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
@@ -1603,7 +1603,7 @@ double or_check_divergence(or_state *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* CalcTorque.F90:121-150 (serial sum, as on one rank)                */
+/* CalcTorque.F90:26-55   (serial sum, as on one rank)                */
 /* ------------------------------------------------------------------ */
 void or_calc_torque(or_state *s, double *inner, double *outer) {
   const int nrm = s->nrm, nr = s->nr, nzm = s->nzm, nthm = s->nthm;

@@ -47,8 +47,8 @@ int launch_status() {
 }
 
 // ---------------------------------------------------------------------------
-// Helmholtz LU tables (SolveImpEqnUpdate_QT.F90:161-187,
-// SolveImpEqnUpdate_QR.F90:310-327, SolveImpEqnUpdate_QZ.F90:158-181)
+// Helmholtz LU tables (SolveImpEqnUpdate_QT.F90:37-63,
+// SolveImpEqnUpdate_QR.F90:27-44, SolveImpEqnUpdate_QZ.F90:35-58)
 // ---------------------------------------------------------------------------
 void write_table(double *T, int nt, int n, std::vector<double> &dl, std::vector<double> &d,
                  std::vector<double> &du, const std::vector<double> &sc) {
@@ -264,7 +264,7 @@ void set_substep(tc_ctx *c, int ns) {
 }
 
 
-// CalcTorque.F90:121-150: normalisation of the masked wall sums
+// CalcTorque.F90:26-55: normalisation of the masked wall sums
 void torque_from_sums(const tc_ctx *c, double slow, double sup, double *inner, double *outer) {
   const tc_params &p = c->p;
   const double reninv = 1.0 / p.ren;
@@ -745,7 +745,7 @@ int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed, doubl
     z = z ^ (z >> 31);
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
   };
-  // CreateInitialConditions.F90:180-182 laminar profile coefficients
+  // CreateInitialConditions.F90:21-23 laminar profile coefficients
   const double eta = p->rint / p->rext;
   const double alam = (p->virotext - eta * eta * p->virotint) / (1 - eta * eta);
   const double blam = (p->virotint - p->virotext) * p->rint * p->rint / (1 - eta * eta);

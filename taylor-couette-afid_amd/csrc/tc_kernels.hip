@@ -49,7 +49,7 @@ DEVI GAcc gacc(const GridDev &g, const double *a, int k, int j, int i) {
   return r;
 }
 
-// ExplicitTermsQT.F90:107-192 at (kc, j, i)
+// ExplicitTermsQT.F90:44-125 at (kc, j, i)
 template <class A>
 DEVI double explicit_qt_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
   const int dkm = kc > 1 ? -1 : 0, dkp = kc < g.nrm ? 1 : 0;  // kmv, kpv clamps
@@ -80,7 +80,7 @@ DEVI double explicit_qt_g(const GridDev &g, const A &T, const A &R, const A &Z, 
   return -(h11 + h12 + h13) + d11qre + d22qt + d11qt + coriol;
 }
 
-// ExplicitTermsQR.F90:247-347 at (kc, j, i), kc = 2..nrm
+// ExplicitTermsQR.F90:46-146 at (kc, j, i), kc = 2..nrm
 template <class A>
 DEVI double explicit_qr_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
   const double udxt = g.dxt * 0.25 * g.usrc[kc];
@@ -111,7 +111,7 @@ DEVI double explicit_qr_g(const GridDev &g, const A &T, const A &R, const A &Z, 
   return -(h21 + h22 + h23) + h11n + d11qr + d22qr + d11qre + coriol;
 }
 
-// ExplicitTermsQZ.F90:395-457 at (kc, j, i)
+// ExplicitTermsQZ.F90:40-102 at (kc, j, i)
 template <class A>
 DEVI double explicit_qz_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
   const int dkm = kc > 1 ? -1 : 0, dkp = kc < g.nrm ? 1 : 0;
@@ -218,7 +218,7 @@ struct ImpArgs {
   double al, ga, ro, dt, alre;
   double vwall_in;  // virotint + vinner_t
   double virotext;
-  double bc1;       // 0.5*al*dt*dvinner_t*am1j(1)/ren (SolveImpEqnUpdate_QT.F90:191)
+  double bc1;       // 0.5*al*dt*dvinner_t*am1j(1)/ren (SolveImpEqnUpdate_QT.F90:67)
   double udxz;      // dxz*al (ImplicitAndUpdateQZ.F90:25)
 };
 
@@ -243,7 +243,7 @@ DEVI double rhs_qt(const GridDev &g, const ImpArgs &a, int k, double qm, double 
   return (a.ga * dqv + a.ro * ruold + a.alre * d33 - dpx) * a.dt;
 }
 
-// Right-hand side row k = 2..nrm of the q_r solve (ImplicitAndUpdateQR.F90:249-271).
+// Right-hand side row k = 2..nrm of the q_r solve (ImplicitAndUpdateQR.F90:41-63).
 DEVI double rhs_qr(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
                    double pr_c, double pr_km, double dphv, double ruold) {
   const double d33 = qp * g.ap2je[k] + q0 * g.ac2je[k] + qm * g.am2je[k];
@@ -276,7 +276,7 @@ DEVI double rhs_qz(const GridDev &g, const ImpArgs &a, int k, double qm, double 
   return (a.ga * qcv + a.ro * hist + a.alre * d33 - dpx) * a.dt;
 }
 
-// ImplicitAndUpdateQT.F90:40-117 + SolveImpEqnUpdate_QT.F90:158-204
+// ImplicitAndUpdateQT.F90:40-117 + SolveImpEqnUpdate_QT.F90:34-78
 __global__ __launch_bounds__(256) void k_implicit_qt(GridDev g, ImpArgs a, double *__restrict__ qt,
                                                      const double *__restrict__ pr,
                                                      const double *__restrict__ dq,
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void k_implicit_qt(GridDev g, ImpArgs a, doubl
   y[(long long)(n - 1) * P + l] = carry;
 }
 
-// ImplicitAndUpdateQR.F90:247-271 + the forward half of SolveImpEqnUpdate_QR:
+// ImplicitAndUpdateQR.F90:39-63 + the forward half of SolveImpEqnUpdate_QR:
 // nr rows with identity rows (zero right-hand side) at both walls.
 __global__ __launch_bounds__(256) void k_implicit_qr(GridDev g, ImpArgs a,
                                                      const double *__restrict__ qr,
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_implicit_qr(GridDev g, ImpArgs a,
   y[(long long)(n - 1) * P + l] = carry;
 }
 
-// ImplicitAndUpdateQZ.F90:36-115 + SolveImpEqnUpdate_QZ.F90:155-186.
+// ImplicitAndUpdateQZ.F90:36-115 + SolveImpEqnUpdate_QZ.F90:32-68.
 __global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a,
                                                      const double *__restrict__ qz,
                                                      const double *__restrict__ pr,
@@ -441,8 +441,8 @@ __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict_
   const int v = icyl[l] + 2 * ocyl[l];
   if (which & 1) usolve_line(lu_view(lut + (size_t)v * nt, nt), g.nrm, 0, qt, yt, l, g.P);
   if (which & 2) {
-    // SolveImpEqnUpdate_QR.F90:346-348 adds rows 2..nrm; walls reset
-    // (ImplicitAndUpdateQR.F90:278-279)
+    // SolveImpEqnUpdate_QR.F90:63-65 adds rows 2..nrm; walls reset
+    // (ImplicitAndUpdateQR.F90:70-71)
     usolve_line(lu_view(lur, nt), g.nr, 1, qr, yr, l, g.P);
     qr[l] = 0.0;
     qr[(long long)(g.nr - 1) * g.P + l] = 0.0;
@@ -781,7 +781,7 @@ void launch_helmholtz_fused(tc_ctx *c) {
 DEVI double divergence_at(const GridDev &g, const double *__restrict__ qt,
                           const double *__restrict__ qr, const double *__restrict__ qz, int kc,
                           int j, int i) {
-  // CalcLocalDivergence.F90:33-35 (also CheckDivergence.F90:83-85)
+  // CalcLocalDivergence.F90:33-35 (also CheckDivergence.F90:34-36)
   const int ip = wrap_p(i, g.nthm), jp = wrap_p(j, g.nzm);
   const long long o = IX(g, kc, j, i);
   return ((qt[IX(g, kc, j, ip)] - qt[o]) * g.dxt + (qr[o + g.P] - qr[o]) * g.udx2m[kc]) * g.usrm[kc] +
@@ -806,7 +806,7 @@ void launch_divergence(tc_ctx *c, double usdt) {
   timer_end(c);
 }
 
-// CorrectVelocity.F90:249-257 (qr walls zeroed by k_zero_wall_qr)
+// CorrectVelocity.F90:32-40 (qr walls zeroed by k_zero_wall_qr)
 DEVI void correct_velocity_at(const GridDev &g, double *__restrict__ qt, double *__restrict__ qr,
                               double *__restrict__ qz, const double *__restrict__ dph, int kc,
                               int j, int i, double dt, double al) {
@@ -819,7 +819,7 @@ DEVI void correct_velocity_at(const GridDev &g, double *__restrict__ qt, double 
   qz[o] = qz[o] - (locdph - dph[IX(g, kc, jm, i)]) * al * dt * g.dxz;
 }
 
-// CorrectPressure.F90:305-315 (amm = apph, be = beta)
+// CorrectPressure.F90:37-46 (amm = apph, be = beta)
 DEVI void correct_pressure_at(const GridDev &g, double *__restrict__ pr,
                               const double *__restrict__ dph, int kc, int j, int i, double be) {
   const int ip = wrap_p(i, g.nthm), im = wrap_m(i, g.nthm);
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(256) void k_correct_both(GridDev g, double *__restr
   if (kc == g.nrm) qr[IX(g, g.nr, j, i)] = 0.0;
 }
 
-// qr(1,:,:) = qr(nr,:,:) = 0 (CorrectVelocity.F90:263-264)
+// qr(1,:,:) = qr(nr,:,:) = 0 (CorrectVelocity.F90:46-47)
 __global__ void k_zero_wall_qr(GridDev g, double *__restrict__ qr) {
   const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= g.P) return;
@@ -952,7 +952,7 @@ DEVI Col column(const GridDev &g) {
   return c;
 }
 
-// CalcMaxCFL.F90:30-43 (WHICH 0) and CheckDivergence.F90:77-89 (WHICH 1)
+// CalcMaxCFL.F90:30-43 (WHICH 0) and CheckDivergence.F90:28-40 (WHICH 1)
 template <int WHICH>
 __global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const double *__restrict__ qt,
                                                          const double *__restrict__ qr,
@@ -981,7 +981,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const doubl
   block_reduce<1>(m, ops, part + blockIdx.x);
 }
 
-// CalcTorque.F90:131-140 masked wall sums
+// CalcTorque.F90:36-45 masked wall sums
 __global__ __launch_bounds__(RED_THREADS) void k_torque(GridDev g, const double *__restrict__ qt,
                                                        const unsigned char *__restrict__ icyl,
                                                        const unsigned char *__restrict__ ocyl,
@@ -1093,9 +1093,9 @@ void launch_velocity_stats(tc_ctx *c) {
 }
 
 // One pass over the velocity for the logging step of main.F90:180-186:
-// CalcMaxCFL (CalcMaxCFL.F90:36-39), CheckDivergence (CheckDivergence.F90:83-86),
+// CalcMaxCFL (CalcMaxCFL.F90:36-39), CheckDivergence (CheckDivergence.F90:34-37),
 // CalcVelocityStats (CalcVelocityStats.F90:35-44) and CalcTorque
-// (CalcTorque.F90:133-138).  v = cfl, |div|, vmax1..3, rms sums 1..3,
+// (CalcTorque.F90:38-43).  v = cfl, |div|, vmax1..3, rms sums 1..3,
 // torque sums inner/outer.
 __global__ __launch_bounds__(RED_THREADS) void k_diagnostics(
     GridDev g, const double *__restrict__ qt, const double *__restrict__ qr,
@@ -1152,18 +1152,24 @@ void launch_diagnostics(tc_ctx *c) {
   timer_end(c);
 }
 
-// Plain streaming copy of one field (nr planes): the reference point for
-// converting hardware byte counters into bytes per launch.
-__global__ __launch_bounds__(256) void k_copy(const double *__restrict__ src, double *__restrict__ dst,
+// Plain streaming copy of one field (nr planes): the achievable-bandwidth
+// reference and the calibration point for hardware byte counters.  One
+// 16-byte element per thread over a grid that covers the field once.
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src,
+                                              double2 *__restrict__ dst, long long n2,
+                                              const double *__restrict__ s1, double *__restrict__ d1,
                                               long long n) {
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x)
-    dst[q] = src[q];
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n2) dst[q] = src[q];
+  if (q == 0 && (n & 1)) d1[n - 1] = s1[n - 1];
 }
 
 void launch_copy(tc_ctx *c, int dst, int src) {
   timer_begin(c, "copy");
-  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, c->f[src], c->f[dst], c->N);
+  const long long n2 = c->N / 2;
+  const unsigned blocks = (unsigned)((n2 + 255) / 256 > 0 ? (n2 + 255) / 256 : 1);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (const double2 *)c->f[src],
+                     (double2 *)c->f[dst], n2, c->f[src], c->f[dst], c->N);
   timer_end(c);
 }
 

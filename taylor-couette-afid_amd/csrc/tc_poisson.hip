@@ -1,4 +1,4 @@
-// Pressure-Poisson solve of SolvePressureCorrection.F90:93-208 on one MI355X.
+// Pressure-Poisson solve of SolvePressureCorrection.F90:100-163 on one MI355X.
 //
 // The reference transforms z (r2c) then theta (c2c), solves one complex
 // radial tridiagonal per (kz, ktheta) and transforms back, with eight pencil
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPla
   }
 }
 
-// SolvePressureCorrection.F90:171-192: radial tridiagonal per (kz, ktheta)
+// SolvePressureCorrection.F90:127-148: radial tridiagonal per (kz, ktheta)
 // mode, sub-diagonal amph, super-diagonal apph and diagonal
 // acph - ak3(kz) - ak1(ktheta)*usrm^2.  Thomas without pivoting (the rows are
 // diagonally dominant).  The singular (0,0) Neumann mode is solved on rows

@@ -14,7 +14,7 @@
 !   --ic ...     seeded perturbed Couette start, projected divergence free;
 !                without it NREAD=0 uses CreateInitialConditions and
 !                NREAD=1 reads continua_{qt,qr,qz}.bin
-! Outputs: torquenu.out (time, inner, outer; e14.6 as CalcTorque.F90:152),
+! Outputs: torquenu.out (time, inner, outer; e14.6 as CalcTorque.F90:59),
 !          torquenu_full.out (same at full precision), vmax.out,
 !          continua_{qt,qr,qz,pr}.bin at exit (raw FP64, (nr,nzm,nthm)).
 !-----------------------------------------------------------------------
@@ -257,7 +257,7 @@ subroutine ReadInputFile
   bou%minstrz = minstrz; bou%multmin = multmin
 end subroutine ReadInputFile
 
-! CreateInitialConditions.F90:180-205 (the reference's default start)
+! CreateInitialConditions.F90:11-49 (the reference's default start)
 subroutine CreateInitialConditions
   use param
   use tc_abi

@@ -1,9 +1,9 @@
 """Writes the golden vectors that pin the oracle's third-party arithmetic.
 
-The reference calls LAPACK ?gttrf/?gttrs (SolveImpEqnUpdate_QT.F90:187,198;
-SolveImpEqnUpdate_QR.F90:327,344; SolveImpEqnUpdate_QZ.F90:181,187;
-SolvePressureCorrection.F90:183-186) and FFTW 3.3.4 unnormalised r2c / c2c /
-c2r transforms (SolvePressureCorrection.F90:105-139,150-204).  Neither ships
+The reference calls LAPACK ?gttrf/?gttrs (SolveImpEqnUpdate_QT.F90:63,74;
+SolveImpEqnUpdate_QR.F90:44,61; SolveImpEqnUpdate_QZ.F90:58,64;
+SolvePressureCorrection.F90:139-141) and FFTW 3.3.4 unnormalised r2c / c2c /
+c2r transforms (SolvePressureCorrection.F90:106-113,156-160).  Neither ships
 with the reference, so the vectors come from scipy.linalg.lapack (the
 reference LAPACK algorithm) and numpy.fft (the same unnormalised DFT) in this
 container.

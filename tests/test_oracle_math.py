@@ -61,7 +61,7 @@ def test_explicit_terms_converge_second_order(tc, oracle):
 
 def test_explicit_terms_solid_body(tc, oracle):
     """qt = Omega rm, qr = qz = 0, no rotation frame: dq = qcap = 0 and dph is
-    the centrifugal term ((qt(k) + qt(k-1))/2)^2 (ExplicitTermsQR.F90:259-260)."""
+    the centrifugal term ((qt(k) + qt(k-1))/2)^2 (ExplicitTermsQR.F90:58-59)."""
     p = params(tc, 16, 12, 17, romeg=0.0)
     o = oracle.Oracle(p)
     rm = o.vec("rm")
@@ -121,7 +121,7 @@ def bessel_alpha(order, ri, ro):
 def decay_rate(tc, oracle, comp, nrm):
     # Re and dt keep beta * max eigenvalue of the Laplacian in the range where
     # the reference's incremental pressure update (be = beta,
-    # CorrectPressure.F90:291) is stable
+    # CorrectPressure.F90:23) is stable
     ren, ri, ro = 100.0, 1.0, 1.5
     p = params(tc, 4, 4, nrm, ren=ren, romeg=0.0, idtv=0, dtmin=1e-3, dtmax=1e-3, istr=0,
                virotint=0.0, virotext=0.0, limitcfl=1e30, tout=1e30)
@@ -159,7 +159,7 @@ def test_bessel_mode_decay(tc, oracle, comp):
 def test_poisson_inverts_divergence_of_gradient(tc, oracle):
     """SolvePressureCorrection is the inverse of D(G .), with D the divergence
     of CalcLocalDivergence.F90:33-35 and G the gradient of CorrectVelocity.F90
-    :252-257, except for the singular (0,0) mode at its last row (gauge)."""
+    :35-40, except for the singular (0,0) mode at its last row (gauge)."""
     for shape in ((16, 12, 17), (30, 18, 21), (24, 20, 33)):
         p = params(tc, *shape)
         o = oracle.Oracle(p)
@@ -190,7 +190,7 @@ def test_poisson_inverts_divergence_of_gradient(tc, oracle):
 
 
 def test_correct_pressure_constant_correction(tc, oracle):
-    """CorrectPressure.F90:308-314 uses apph for both radial neighbours and
+    """CorrectPressure.F90:40-46 uses apph for both radial neighbours and
     be = beta: a constant phi = c gives pr = c (1 - beta (apph - amph))."""
     p = params(tc, 8, 8, 13)
     o = oracle.Oracle(p)
@@ -236,7 +236,7 @@ def test_qz_wall_history_is_overwritten(tc, oracle):
 
 @pytest.mark.parametrize("nrm", [33])
 def test_couette_torque_converges(tc, oracle, nrm):
-    """CalcTorque.F90:121-150 on the laminar Couette profile v = A r + B / r
+    """CalcTorque.F90:26-55 on the laminar Couette profile v = A r + B / r
     with the wall velocities VIROTINT, VIROTEXT of the boundary conditions
     (ImplicitAndUpdateQT.F90:57,105) tends to 1.  The wall gradient is a
     one-sided difference weighted with rc(2)^3 (rc(nrm)^3), so the

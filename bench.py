@@ -58,8 +58,9 @@ def kernel_bytes(p):
         "implicit_qz": 8 * R,        # qz pr qcap ru3 y | ru3 y qz
         "divergence": 4 * R,         # qt qr qz -> dph
         "fft_theta_fwd": R + C,      # dph -> spec
+        "div_fft_theta_fwd": 3 * R + C,  # qt qr qz -> spec (divergence fused)
         "fft_z_fwd": 2 * C, "fft_z_inv": 2 * C,
-        "poisson_tridiag": 2 * C + C,  # spec in/out, multipliers out/in (8 B each)
+        "poisson_tridiag": 2 * C,    # spec in place
         "fft_theta_inv": C + R,      # spec -> dph
         "correct_both": 9 * R,       # dph qt qr qz pr | qt qr qz pr
         "correct_velocity": 7 * R, "correct_pressure": 3 * R,

@@ -197,7 +197,6 @@ void destroy_ctx(tc_ctx *c) {
   for (int q = 0; q < TC_NFIELDS; q++)
     if (c->f[q]) hipFree(c->f[q]);
   if (c->spec) hipFree(c->spec);
-  if (c->cpw) hipFree(c->cpw);
   if (c->gvbuf) hipFree(c->gvbuf);
   if (c->mask) hipFree(c->mask);
   if (c->lu) hipFree(c->lu);
@@ -234,8 +233,12 @@ int run_substep(tc_ctx *c) {
     tc::launch_implicit_qr(c);
     tc::launch_implicit_qz(c);
   }
-  tc::launch_divergence(c, 1.0 / (c->al * c->dt));
-  tc::launch_poisson(c);
+  if (c->fused) {
+    tc::launch_poisson(c, true, 1.0 / (c->al * c->dt));
+  } else {
+    tc::launch_divergence(c, 1.0 / (c->al * c->dt));
+    tc::launch_poisson(c);
+  }
   if (c->fused) {
     tc::launch_correct_both(c);
   } else {
@@ -336,9 +339,11 @@ int tc_create(const tc_params *p, tc_ctx **out) {
     const char *et = std::getenv("TC_FFT_T_LDS"), *ez = std::getenv("TC_FFT_Z_LDS");
     const char *ezt = std::getenv("TC_FFT_Z_NT");
     const int zt = (ezt && std::atoi(ezt) == 256) ? 256 : 512;
-    const int lt = et ? std::atoi(et) : 16384, lz = ez ? std::atoi(ez) : 65536;
+    const int lt = et ? std::atoi(et) : 32768, lz = ez ? std::atoi(ez) : 65536;
     if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, lt, 256);
     if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, lz, zt);
+    const char *epad = std::getenv("TC_FFT_PAD");
+    if (epad && std::atoi(epad) == 0) c->plan_t.padsh = c->plan_z.padsh = 31;
   }
   if (!e.empty()) {
     delete c;
@@ -383,7 +388,6 @@ int tc_create(const tc_params *p, tc_ctx **out) {
   }
   const size_t M = (size_t)c->nzm * c->sstr;
   TRY(dev_alloc(c, (void **)&c->spec, sizeof(double2) * M * c->nrm));
-  TRY(dev_alloc(c, (void **)&c->cpw, sizeof(double) * M * c->nrm));
   // grid vectors
   {
     const tc::Grid &g = c->g;
@@ -460,6 +464,11 @@ int tc_create(const tc_params *p, tc_ctx **out) {
     // lines per thread of the fused Helmholtz kernel (TC_HELM_LPT overrides)
     const char *lp = std::getenv("TC_HELM_LPT");
     c->helm_lpt = lp ? std::max(1, std::min(2, std::atoi(lp))) : tc::choose_helm_lpt(c);
+    // rows per thread of the radial Poisson solve (TC_POISSON_PQ overrides)
+    const char *pq = std::getenv("TC_POISSON_PQ");
+    c->poisson_pq = (pq && std::atoi(pq) == 13) ? 13 : 26;
+    const char *dfs = std::getenv("TC_DIVFFT_STAGE");
+    c->divfft_stage = dfs ? std::atoi(dfs) : 0;
   }
   TRY(refresh_tables(c));
   HTRY(hipStreamSynchronize(c->stream));

@@ -23,6 +23,8 @@ static void init_time_march(const tc_params &p, Grid &g) {
 
 std::string build_grid(const tc_params &p, Grid &g) {
   if (p.nthm < 2 || p.nzm < 2 || p.nrm < 3) return "grid too small (need NTM>=2, NZM>=2, NRM>=3)";
+  // the radial Poisson solve holds at most 26 rows per thread, 512 threads per line
+  if (p.nrm > 26 * 512) return "grid too large (need NRM <= 13312)";
   if (p.nsst != 1 && p.nsst != 3) return "NSST must be 1 (AB2) or 3 (RK3)";
   if (p.istr != 0 && p.istr != 4 && p.istr != 6) return "ISTR must be 0, 4 or 6";
   if (p.ren <= 0) return "1/NU must be positive";

@@ -33,6 +33,7 @@ struct FFTPlan {
   int lines = 1;  // FFT lines per 256-thread block
   int me = 8;     // points per thread per pass (n <= me * nt/lines)
   int nt = 256;   // threads per block
+  int padsh = 3;  // one LDS padding slot per 2^padsh points (31: none)
 };
 
 // Helmholtz LU tables: per (substep slot, component, mask variant) a block of
@@ -55,7 +56,6 @@ struct tc_ctx {
   int device = 0;
   double *f[TC_NFIELDS] = {nullptr};
   double2 *spec = nullptr;   // nrm x nzm x sstr complex, [r][kz][ktheta]
-  double *cpw = nullptr;     // nrm x nzm x sstr real: Thomas multipliers
   double *gvbuf = nullptr;   // device grid vectors
   tc::GridDev gd;
   unsigned char *mask = nullptr;  // device [2][nzm*nthm] (icyl, ocyl), [j][i]
@@ -80,6 +80,8 @@ struct tc_ctx {
   double time = 0, dt = 0, beta = 0, al = 0, ga = 0, ro = 0, vinner_t = 0, dvinner_t = 0;
   int ns = 1;
   bool fused = true;
+  int divfft_stage = 0;  // fused divergence FFT: stage the divergence rows in LDS first
+  int poisson_pq = 26;  // rows per thread of the radial Poisson solve (13 or 26)
   int helm_lpt = 1;  // radial lines per thread in the fused Helmholtz kernel (1 or 2)
   // CFL of the current velocity, when known (set by the logging pass of tc_step)
   bool cfl_valid = false;
@@ -120,7 +122,9 @@ void launch_copy(tc_ctx *c, int dst, int src);
 // Poisson solve (tc_poisson.hip)
 std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes, int nt);
 void fill_twiddles(int n, std::vector<double> &tw);
-void launch_poisson(tc_ctx *c);
+// with_divergence: compute dph = div(q)*usdt inside the forward theta FFT
+// instead of reading it (the fused time marcher)
+void launch_poisson(tc_ctx *c, bool with_divergence = false, double usdt = 0.0);
 
 // timing helpers (tc_abi.cpp)
 void timer_begin(tc_ctx *c, const char *name);

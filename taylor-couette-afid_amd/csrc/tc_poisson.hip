@@ -29,6 +29,9 @@ DEVI double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b
 // multiply by (s * i), s = +-1
 DEVI double2 cmul_si(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
 
+// LDS slot of element x with one padding slot after every 2^sh elements
+DEVI int pidx(int x, int sh) { return x + (x >> sh); }
+
 // In-register DFT of length R with exponent sign sg (-1 forward, +1 backward).
 template <int R>
 DEVI void dft(double2 *v, double sg);
@@ -99,15 +102,20 @@ DEVI void dft<8>(double2 *v, double sg) {
   }
 }
 
-// One Stockham pass of radix R over a line of n points in LDS, element e at
-// buf[e*stride].  In place: every thread reads the inputs of its butterflies
-// into registers, barrier, writes, barrier.  Ns is the product of the radices
-// of the earlier passes.  ME bounds the points a thread holds per pass
+// One Stockham pass of radix R over a line of n points.  Ns is the product
+// of the radices of the earlier passes; butterfly b reads points b + r*n/R
+// and writes points (b/Ns)*Ns*R + b%Ns + r*Ns.  Points live in LDS, except
+// that the first pass may read them from memory
+// through ld(e) (FROM_G) and the last pass may write them to memory through
+// st(e, value) (TO_G), which saves one LDS round trip and one barrier at
+// each end of the transform.  Point e of the line is at LDS slot
+// pidx(off + e*stride): one padding slot per 2^sh slots spreads the strided
+// Stockham writes over the LDS banks.  ME bounds the points a thread holds per pass
 // (n <= ME*T), so the register arrays have a compile-time size.  All threads
 // of the block must call.
-template <int R, int ME>
-DEVI void fft_pass(double2 *buf, int stride, int n, int Ns, int t, int T,
-                   const double2 *__restrict__ tw, double sg) {
+template <int R, int ME, bool FROM_G, bool TO_G, class Load, class Store>
+DEVI void fft_pass(double2 *buf, int off, int stride, int sh, int n, int Ns, int t, int T,
+                   const double2 *__restrict__ tw, double sg, Load &ld, Store &st) {
   constexpr int MB = (ME + R - 1) / R;
   const int nb = n / R;
   const int twstep = n / (Ns * R);
@@ -119,7 +127,9 @@ DEVI void fft_pass(double2 *buf, int stride, int n, int Ns, int t, int T,
       const int k = b % Ns;
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        double2 x = buf[(b + r * nb) * stride];
+        double2 x;
+        if constexpr (FROM_G) x = ld(b + r * nb);
+        else x = buf[pidx(off + (b + r * nb) * stride, sh)];
         if (r > 0 && k > 0) {
           double2 w = tw[r * k * twstep];
           w.y = (sg < 0) ? w.y : -w.y;
@@ -130,7 +140,7 @@ DEVI void fft_pass(double2 *buf, int stride, int n, int Ns, int t, int T,
       dft<R>(v[q], sg);
     }
   }
-  __syncthreads();
+  if constexpr (!FROM_G) __syncthreads();
 #pragma unroll
   for (int q = 0; q < MB; q++) {
     const int b = t + q * T;
@@ -138,51 +148,115 @@ DEVI void fft_pass(double2 *buf, int stride, int n, int Ns, int t, int T,
       const int k = b % Ns;
       const int base = (b / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[(base + r * Ns) * stride] = v[q][r];
+      for (int r = 0; r < R; r++) {
+        if constexpr (TO_G) st(base + r * Ns, v[q][r]);
+        else buf[pidx(off + (base + r * Ns) * stride, sh)] = v[q][r];
+      }
     }
   }
-  __syncthreads();
+  if constexpr (!TO_G) __syncthreads();
 }
 
-template <int ME>
-DEVI void fft_line(double2 *buf, int stride, const FFTPlan &pl, int t, int T,
-                   const double2 *__restrict__ tw, double sg) {
-  int Ns = 1;
-  for (int s = 0; s < pl.nst; s++) {
-    const int R = pl.radix[s];
-    switch (R) {
-      case 8: fft_pass<8, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
-      case 4: fft_pass<4, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
-      case 2: fft_pass<2, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
-      case 3: fft_pass<3, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
-      default: fft_pass<5, ME>(buf, stride, pl.n, Ns, t, T, tw, sg); break;
-    }
-    Ns *= R;
+template <int ME, bool FROM_G, bool TO_G, class Load, class Store>
+DEVI void fft_pass_r(int R, double2 *buf, int off, int stride, int sh, int n, int Ns, int t,
+                     int T, const double2 *__restrict__ tw, double sg, Load &ld, Store &st) {
+  switch (R) {
+    case 8: fft_pass<8, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
+    case 4: fft_pass<4, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
+    case 2: fft_pass<2, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
+    case 3: fft_pass<3, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
+    default: fft_pass<5, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
   }
+}
+
+// A whole line: with FROM_G the input comes from ld(e), otherwise from LDS;
+// with TO_G the output goes to st(e, value), otherwise to LDS (in natural
+// order, followed by a barrier).
+template <int ME, bool FROM_G, bool TO_G, class Load, class Store>
+DEVI void fft_line(double2 *buf, int off, int stride, const FFTPlan &pl, int t, int T,
+                   const double2 *__restrict__ tw, double sg, Load ld, Store st) {
+  const int n = pl.n, ns = pl.nst, sh = pl.padsh;
+  if (ns == 1) {
+    fft_pass_r<ME, FROM_G, TO_G>(pl.radix[0], buf, off, stride, sh, n, 1, t, T, tw, sg, ld, st);
+    return;
+  }
+  fft_pass_r<ME, FROM_G, false>(pl.radix[0], buf, off, stride, sh, n, 1, t, T, tw, sg, ld, st);
+  int Ns = pl.radix[0];
+  for (int s = 1; s < ns - 1; s++) {
+    fft_pass_r<ME, false, false>(pl.radix[s], buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st);
+    Ns *= pl.radix[s];
+  }
+  fft_pass_r<ME, false, TO_G>(pl.radix[ns - 1], buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st);
 }
 
 // r2c along theta for row pairs: rows 2p and 2p+1 of the real input become
 // the real and imaginary parts of one complex FFT.  Row r of the real input
-// is at in + r*nthm, row r of the half spectrum at spec + r*nth2.
-template <int ME>
-__global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict__ in,
-                                                       double2 *__restrict__ spec, FFTPlan pl,
-                                                       const double2 *__restrict__ tw,
-                                                       long long nrows, int nth2, int sstr) {
+// is at in + r*nthm, row r of the half spectrum at spec + r*sstr.
+// With DIV the real rows are not read from memory: the first pass computes
+// them as the local divergence of CalcLocalDivergence.F90:33-37 (row
+// r = (kc-1)*nzm + j), so dph is neither written nor re-read.
+struct DivIn {
+  const double *qt, *qr, *qz;
+  double usdt;
+};
+
+template <int ME, bool DIV, bool STAGE = false>
+DEVI void theta_fwd_body(const double *__restrict__ in, const DivIn &dv, const GridDev &g,
+                         double2 *__restrict__ spec, const FFTPlan &pl,
+                         const double2 *__restrict__ tw, long long nrows, int nth2, int sstr) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = 256 / L;
   const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
   const long long pair = (long long)blockIdx.x * L + ln;
   const long long r0 = 2 * pair, r1 = r0 + 1;
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
-  double2 *buf = lds + ln * n;
-  const double *a = in + r0 * n, *b = in + r1 * n;
-  for (int e = t; e < n; e += T) buf[e] = make_double2(v0 ? a[e] : 0.0, v1 ? b[e] : 0.0);
-  __syncthreads();
-  fft_line<ME>(buf, 1, pl, t, T, tw, -1.0);
+  double2 *buf = lds;
+  const int off = ln * n, sh = pl.padsh;
+  if constexpr (DIV) {
+    // per row: qt, qr(kc), qr(kc+1) = qr(kc) + P, qz(j), qz(j+1) and the metric
+    struct Row {
+      const double *qt, *qr, *qz, *qzp;  // qt, qr(kc), qz(j), qz(j+1) rows
+      double a, b;                       // usrm(kc), udx2m(kc)
+    };
+    auto row = [&](long long r) {
+      Row w;
+      const long long rr = r < nrows ? r : 0;
+      const int i = (int)(rr / g.nzm), j = (int)(rr - (long long)i * g.nzm);
+      const int jp = (j + 1 == g.nzm) ? 0 : j + 1;
+      w.qt = dv.qt + rr * n;
+      w.qr = dv.qr + rr * n;
+      w.qz = dv.qz + rr * n;
+      w.qzp = dv.qz + ((long long)i * g.nzm + jp) * n;
+      w.a = g.usrm[i + 1];
+      w.b = g.udx2m[i + 1];
+      return w;
+    };
+    const Row w0 = row(r0), w1 = row(r1);
+    const long long P = g.P;
+    auto div = [&](const Row &w, int e) {
+      const int ep = (e + 1 == n) ? 0 : e + 1;
+      return (((w.qt[ep] - w.qt[e]) * g.dxt + (w.qr[e + P] - w.qr[e]) * w.b) * w.a +
+              (w.qzp[e] - w.qz[e]) * g.dxz) *
+             dv.usdt;
+    };
+    auto ld = [&](int e) { return make_double2(v0 ? div(w0, e) : 0.0, v1 ? div(w1, e) : 0.0); };
+    auto st = [&](int, double2) {};
+    if constexpr (STAGE) {
+      for (int e = t; e < n; e += T) buf[pidx(off + e, sh)] = ld(e);
+      __syncthreads();
+      fft_line<ME, false, false>(buf, off, 1, pl, t, T, tw, -1.0, ld, st);
+    } else {
+      fft_line<ME, true, false>(buf, off, 1, pl, t, T, tw, -1.0, ld, st);
+    }
+  } else {
+    const double *a = in + r0 * n, *b = in + r1 * n;
+    auto ld = [&](int e) { return make_double2(v0 ? a[e] : 0.0, v1 ? b[e] : 0.0); };
+    auto st = [&](int, double2) {};
+    fft_line<ME, true, false>(buf, off, 1, pl, t, T, tw, -1.0, ld, st);
+  }
   for (int k = t; k < nth2; k += T) {
-    const double2 z = buf[k];
-    const double2 zc = buf[(n - k) % n];
+    const double2 z = buf[pidx(off + k, sh)];
+    const double2 zc = buf[pidx(off + (n - k) % n, sh)];
     // A = (Z + conj(Zc))/2, B = (Z - conj(Zc))/(2i)
     const double2 A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
     const double2 B = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
@@ -191,8 +265,29 @@ __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict_
   }
 }
 
+template <int ME>
+__global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict__ in,
+                                                       double2 *__restrict__ spec, FFTPlan pl,
+                                                       const double2 *__restrict__ tw,
+                                                       long long nrows, int nth2, int sstr) {
+  const DivIn dv{};
+  const GridDev g{};
+  theta_fwd_body<ME, false>(in, dv, g, spec, pl, tw, nrows, nth2, sstr);
+}
+
+template <int ME, bool STAGE>
+__global__ __launch_bounds__(256) void k_div_fft_theta_fwd(DivIn dv, GridDev g,
+                                                           double2 *__restrict__ spec, FFTPlan pl,
+                                                           const double2 *__restrict__ tw,
+                                                           long long nrows, int nth2, int sstr) {
+  theta_fwd_body<ME, true, STAGE>(nullptr, dv, g, spec, pl, tw, nrows, nth2, sstr);
+}
+
 // c2r along theta for row pairs; the imaginary parts of ktheta = 0 and
-// (n even) ktheta = n/2 are ignored, as a c2r transform does.
+// (n even) ktheta = n/2 are ignored, as a c2r transform does.  Each half
+// spectrum value is read once: Z[k] = A[k] + i B[k] and
+// Z[n-k] = conj(A[k]) + i conj(B[k]) go to LDS together; the last pass
+// writes the two real rows straight to memory.
 template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict__ spec,
                                                        double *__restrict__ out, FFTPlan pl,
@@ -204,29 +299,25 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
   const long long pair = (long long)blockIdx.x * L + ln;
   const long long r0 = 2 * pair, r1 = r0 + 1;
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
-  double2 *buf = lds + ln * n;
-  for (int k = t; k < n; k += T) {
-    const bool lo = (k < nth2);
-    const int kk = lo ? k : n - k;
-    double2 A = v0 ? spec[r0 * sstr + kk] : make_double2(0.0, 0.0);
-    double2 B = v1 ? spec[r1 * sstr + kk] : make_double2(0.0, 0.0);
-    if (!lo) {
-      A.y = -A.y;
-      B.y = -B.y;
-    }
+  double2 *buf = lds;
+  const int off = ln * n, sh = pl.padsh;
+  for (int k = t; k < nth2; k += T) {
+    double2 A = v0 ? spec[r0 * sstr + k] : make_double2(0.0, 0.0);
+    double2 B = v1 ? spec[r1 * sstr + k] : make_double2(0.0, 0.0);
     if (k == 0 || 2 * k == n) {
       A.y = 0.0;
       B.y = 0.0;
     }
-    buf[k] = make_double2(A.x - B.y, A.y + B.x);
+    buf[pidx(off + k, sh)] = make_double2(A.x - B.y, A.y + B.x);
+    if (k > 0 && 2 * k != n) buf[pidx(off + n - k, sh)] = make_double2(A.x + B.y, B.x - A.y);
   }
   __syncthreads();
-  fft_line<ME>(buf, 1, pl, t, T, tw, 1.0);
-  for (int e = t; e < n; e += T) {
-    const double2 z = buf[e];
+  auto ld = [&](int) { return make_double2(0.0, 0.0); };
+  auto st = [&](int e, double2 z) {
     if (v0) out[r0 * n + e] = z.x;
     if (v1) out[r1 * n + e] = z.y;
-  }
+  };
+  fft_line<ME, false, true>(buf, off, 1, pl, t, T, tw, 1.0, ld, st);
 }
 
 // Workgroups are dispatched round-robin over the XCDs; hand each XCD a
@@ -251,89 +342,212 @@ __global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPla
   const int c = threadIdx.x % L, t = threadIdx.x / L;
   const int id = xcd_tile(blockIdx.x, gridDim.x);
   const int c0 = (id % ntile) * L;
-  double2 *plane = spec + (long long)(id / ntile) * n * sstr;
-  for (int e = threadIdx.x; e < L * n; e += NT) {
-    const int z = e / L, cc = e - z * L;
-    lds[e] = (c0 + cc < nth2) ? plane[(long long)z * sstr + c0 + cc] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  fft_line<ME>(lds + c, L, pl, t, T, tw, sg);
-  for (int e = threadIdx.x; e < L * n; e += NT) {
-    const int z = e / L, cc = e - z * L;
-    if (c0 + cc < nth2) plane[(long long)z * sstr + c0 + cc] = lds[e];
-  }
+  double2 *col = spec + (long long)(id / ntile) * n * sstr + c0 + c;
+  const bool ok = c0 + c < nth2;
+  auto ld = [&](int z) { return ok ? col[(long long)z * sstr] : make_double2(0.0, 0.0); };
+  auto st = [&](int z, double2 v) {
+    if (ok) col[(long long)z * sstr] = v;
+  };
+  fft_line<ME, true, true>(lds, c, L, pl, t, T, tw, sg, ld, st);
 }
 
 // SolvePressureCorrection.F90:127-148: radial tridiagonal per (kz, ktheta)
-// mode, sub-diagonal amph, super-diagonal apph and diagonal
-// acph - ak3(kz) - ak1(ktheta)*usrm^2.  Thomas without pivoting (the rows are
-// diagonally dominant).  The singular (0,0) Neumann mode is solved on rows
-// 1..nrm-1 with phi(nrm) = 0.
-__global__ __launch_bounds__(256) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
-                                                         double *__restrict__ cpw,
-                                                         double inv_norm) {
-  constexpr int CH = 8;  // rows loaded ahead of the recurrence
+// mode, sub-diagonal amph (rows 2..nrm), super-diagonal apph (rows 1..nrm-1)
+// and diagonal acph - ak3(kz) - ak1(ktheta)*usrm^2.
+//
+// Partition method, one read and one write of the spectrum: the nrm rows of a
+// line are split over PT threads (threadIdx.y) of at most PQ rows each, held
+// in registers.  Each thread eliminates its rows down and up so that every
+// row couples only to the partition's first and last unknowns; those 2*PT
+// boundary unknowns form a tridiagonal system with unit diagonal that one
+// thread per line solves in LDS; every thread then recovers its interior
+// rows.  No pivoting: the rows are diagonally dominant.
+//
+// The singular (0,0) Neumann mode is solved on rows 1..nrm-1 with
+// phi(nrm) = 0.  Its partitions are nearly singular, so block 0 stages that
+// one line in LDS and solves it with a serial Thomas sweep instead.
+//   blockDim = (LB lines, PT partitions)
+//   LDS = LB*2PT*4 doubles (reduced systems) + 3*nrm doubles (gauge line)
+template <int PQ>  // max rows per thread
+__global__ __launch_bounds__(512) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
+                                                      double inv_norm) {
+  extern __shared__ double red[];  // [LB][2*PT][4]: A, C, D.x, D.y
+  const int LB = blockDim.x, PT = blockDim.y;
+  const int lx = threadIdx.x, pp = threadIdx.y;
+  const int tid = pp * LB + lx, nthr = LB * PT;
   const long long M = (long long)g.nzm * g.sstr;
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  const int kz = (int)(m / g.sstr), kt = (int)(m - (long long)kz * g.sstr);
-  if (kt >= g.nth2) return;  // row padding
-  const double sz = g.ak3[kz + 1], st = g.ak1[kt + 1];
-  const int nn = (m == 0) ? g.nrm - 1 : g.nrm;
-  // forward elimination
-  double b = g.acph[1] - sz - st * (g.usrm[1] * g.usrm[1]);
-  double rden = 1.0 / b;
-  double cp = g.apph[1] * rden;
-  double2 d = spec[m];
-  double2 dp = make_double2(d.x * inv_norm * rden, d.y * inv_norm * rden);
-  cpw[m] = cp;
-  spec[m] = dp;
-  for (int k0 = 2; k0 <= nn; k0 += CH) {
-    double2 dv[CH];
+  const long long m = (long long)blockIdx.x * LB + lx;
+  const int n = g.nrm;
+  double2 *gline = (double2 *)(red + (size_t)LB * 2 * PT * 4);  // gauge line, n values
+  double *gcp = (double *)(gline + n);                          // its multipliers
+  const bool gblock = (blockIdx.x == 0);
+  if (gblock)
+    for (int k = tid; k < n; k += nthr) {
+      const double2 v = spec[(long long)k * M];
+      gline[k] = make_double2(v.x * inv_norm, v.y * inv_norm);
+    }
+  bool valid = m < M && m != 0;
+  int kz = 0, kt = 0;
+  if (valid) {
+    kz = (int)(m / g.sstr);
+    kt = (int)(m - (long long)kz * g.sstr);
+    valid = kt < g.nth2;
+  }
+  const double sz = valid ? g.ak3[kz + 1] : 0.0, st = valid ? g.ak1[kt + 1] : 0.0;
+  const int lo = (int)(((long long)pp * n) / PT), S = (int)(((long long)(pp + 1) * n) / PT) - lo;
+  // row k (1-based) of this line: a x(k-1) + b x(k) + c x(k+1) = d
+  auto coef = [&](int k, double &a, double &b, double &c) {
+    a = (k > 1) ? g.amph[k] : 0.0;
+    c = (k < n) ? g.apph[k] : 0.0;
+    b = g.acph[k] - sz - st * (g.usrm[k] * g.usrm[k]);
+    if (!valid) { a = 0.0; b = 1.0; c = 0.0; }
+  };
+  double ar[PQ], cr[PQ];
+  double2 d[PQ];
 #pragma unroll
-    for (int u = 0; u < CH; u++)
-      if (k0 + u <= nn) dv[u] = spec[(long long)(k0 + u - 1) * M + m];
+  for (int i = 0; i < PQ; i++) {
+    d[i] = make_double2(0.0, 0.0);
+    if (i < S && valid) {
+      const double2 v = spec[(long long)(lo + i) * M + m];
+      d[i] = make_double2(v.x * inv_norm, v.y * inv_norm);
+    }
+  }
+  // downward: rows i >= 1 become a_i x_0 + x_i + c_i x_(i+1) = d_i
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int k = k0 + u;
-      if (k <= nn) {
-        const long long o = (long long)(k - 1) * M + m;
-        const double a = g.amph[k];
-        b = g.acph[k] - sz - st * (g.usrm[k] * g.usrm[k]);
-        rden = 1.0 / (b - a * cp);
-        dp = make_double2((dv[u].x * inv_norm - a * dp.x) * rden, (dv[u].y * inv_norm - a * dp.y) * rden);
-        spec[o] = dp;
-        if (k < nn) {
-          cp = g.apph[k] * rden;
-          cpw[o] = cp;
-        }
+  for (int i = 0; i < PQ; i++) {
+    if (i < S) {
+      double a, b, c;
+      coef(lo + i + 1, a, b, c);
+      if (i < 2) {
+        const double r = 1.0 / b;
+        ar[i] = a * r;
+        cr[i] = c * r;
+        d[i] = make_double2(d[i].x * r, d[i].y * r);
+      } else {
+        const double r = 1.0 / (b - a * cr[i - 1]);
+        d[i] = make_double2((d[i].x - a * d[i - 1].x) * r, (d[i].y - a * d[i - 1].y) * r);
+        ar[i] = -a * ar[i - 1] * r;
+        cr[i] = c * r;
       }
     }
   }
-  // back substitution
-  double2 x = dp;
-  for (int k0 = nn - 1; k0 >= 1; k0 -= CH) {
-    double2 yv[CH];
-    double cv[CH];
+  // upward: rows 1 <= i <= S-2 become a_i x_0 + x_i + c_i x_(S-1) = d_i
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int k = k0 - u;
-      if (k >= 1) {
-        const long long o = (long long)(k - 1) * M + m;
-        yv[u] = spec[o];
-        cv[u] = cpw[o];
-      }
+  for (int i = PQ - 3; i >= 1; i--) {
+    if (i <= S - 3) {
+      d[i] = make_double2(d[i].x - cr[i] * d[i + 1].x, d[i].y - cr[i] * d[i + 1].y);
+      ar[i] = ar[i] - cr[i] * ar[i + 1];
+      cr[i] = -cr[i] * cr[i + 1];
     }
+  }
+  // first row: A x_(-1) + x_0 + C x_(S-1) = D
+  double A0 = ar[0], C0 = cr[0];
+  double2 D0 = d[0];
+  if (S >= 3) {
+    const double r = 1.0 / (1.0 - cr[0] * ar[1]);
+    D0 = make_double2((d[0].x - cr[0] * d[1].x) * r, (d[0].y - cr[0] * d[1].y) * r);
+    A0 = ar[0] * r;
+    C0 = -cr[0] * cr[1] * r;
+  }
+  double aL = 0.0, cL = 0.0;
+  double2 dL = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int k = k0 - u;
-      if (k >= 1) {
-        x = make_double2(yv[u].x - cv[u] * x.x, yv[u].y - cv[u] * x.y);
-        spec[(long long)(k - 1) * M + m] = x;
+  for (int i = 1; i < PQ; i++)
+    if (i == S - 1) { aL = ar[i]; cL = cr[i]; dL = d[i]; }
+  double *rl = red + (size_t)lx * (2 * PT) * 4;
+  rl[(2 * pp) * 4 + 0] = A0;
+  rl[(2 * pp) * 4 + 1] = C0;
+  rl[(2 * pp) * 4 + 2] = D0.x;
+  rl[(2 * pp) * 4 + 3] = D0.y;
+  rl[(2 * pp + 1) * 4 + 0] = aL;
+  rl[(2 * pp + 1) * 4 + 1] = cL;
+  rl[(2 * pp + 1) * 4 + 2] = dL.x;
+  rl[(2 * pp + 1) * 4 + 3] = dL.y;
+  __syncthreads();
+  if (pp == 0) {
+    // unit-diagonal tridiagonal system of the 2*PT boundary unknowns
+    // (row r+1 is read before row r is written: the reads do not wait on
+    // the stores)
+    const int R = 2 * PT;
+    double cprev = 0.0, dxp = 0.0, dyp = 0.0;
+    double4 nx = *(const double4 *)(rl);
+    for (int r = 0; r < R; r++) {
+      const double4 cur = nx;
+      if (r + 1 < R) nx = *(const double4 *)(rl + (r + 1) * 4);
+      const double q = 1.0 / (1.0 - cur.x * cprev);
+      cprev = cur.y * q;
+      dxp = (cur.z - cur.x * dxp) * q;
+      dyp = (cur.w - cur.x * dyp) * q;
+      rl[r * 4 + 1] = cprev;
+      rl[r * 4 + 2] = dxp;
+      rl[r * 4 + 3] = dyp;
+    }
+    double xx = dxp, xy = dyp;
+    double4 pv = (R >= 2) ? *(const double4 *)(rl + (R - 2) * 4) : make_double4(0, 0, 0, 0);
+    for (int r = R - 2; r >= 0; r--) {
+      const double4 cur = pv;
+      if (r >= 1) pv = *(const double4 *)(rl + (r - 1) * 4);
+      xx = cur.z - cur.y * xx;
+      xy = cur.w - cur.y * xy;
+      rl[r * 4 + 2] = xx;
+      rl[r * 4 + 3] = xy;
+    }
+  }
+  if (gblock && tid == nthr - 1) {
+    // gauge line: Thomas on rows 1..nrm-1 (b = acph there), phi(nrm) = 0
+    const int nn = n - 1;
+    double b = g.acph[1] - g.ak3[1] - g.ak1[1] * (g.usrm[1] * g.usrm[1]);
+    double rden = 1.0 / b;
+    double cp = g.apph[1] * rden;
+    double2 dp = make_double2(gline[0].x * rden, gline[0].y * rden);
+    gcp[0] = cp;
+    gline[0] = dp;
+    for (int k = 2; k <= nn; k++) {
+      const double a = g.amph[k];
+      b = g.acph[k] - g.ak3[1] - g.ak1[1] * (g.usrm[k] * g.usrm[k]);
+      rden = 1.0 / (b - a * cp);
+      dp = make_double2((gline[k - 1].x - a * dp.x) * rden, (gline[k - 1].y - a * dp.y) * rden);
+      gline[k - 1] = dp;
+      cp = g.apph[k] * rden;
+      gcp[k - 1] = cp;
+    }
+    double2 x = dp;
+    for (int k = nn - 1; k >= 1; k--) {
+      x = make_double2(gline[k - 1].x - gcp[k - 1] * x.x, gline[k - 1].y - gcp[k - 1] * x.y);
+      gline[k - 1] = x;
+    }
+    gline[n - 1] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (valid) {
+    const double2 x0 = make_double2(rl[(2 * pp) * 4 + 2], rl[(2 * pp) * 4 + 3]);
+    const double2 xl = make_double2(rl[(2 * pp + 1) * 4 + 2], rl[(2 * pp + 1) * 4 + 3]);
+#pragma unroll
+    for (int i = 0; i < PQ; i++) {
+      if (i < S) {
+        double2 x;
+        if (i == 0) x = x0;
+        else if (i == S - 1) x = xl;
+        else
+          x = make_double2(d[i].x - ar[i] * x0.x - cr[i] * xl.x, d[i].y - ar[i] * x0.y - cr[i] * xl.y);
+        spec[(long long)(lo + i) * M + m] = x;
       }
     }
   }
-  if (m == 0) spec[(long long)(g.nrm - 1) * M] = make_double2(0.0, 0.0);
+  if (gblock)
+    for (int k = tid; k < n; k += nthr) spec[(long long)k * M] = gline[k];
+}
+
+// Launch shape of k_poisson_tridiag for a line of n rows: PT partitions of at
+// most PQ rows (and at least 2), LB lines per block with LB*PT <= 512
+// (<= 256 for PQ = 13, so that three blocks share a CU).
+void poisson_part_shape(int n, int PQ, int &pt, int &lb) {
+  pt = (n + PQ - 1) / PQ;
+  if (pt < 1) pt = 1;
+  while (pt > 1 && n < 2 * pt) pt--;
+  const int maxt = (PQ <= 13) ? 256 : 512;
+  lb = 64;
+  while (lb > 1 && lb * pt > maxt) lb /= 2;
 }
 
 std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes, int nt) {
@@ -380,9 +594,22 @@ void fill_twiddles(int n, std::vector<double> &tw) {
   }
 }
 
+// LDS of one block: pl.lines lines of pl.n points plus the padding slots
+static size_t fft_lds_bytes(const FFTPlan &pl) {
+  const long long last = (long long)pl.lines * pl.n - 1;
+  return (size_t)(last + (last >> pl.padsh) + 1) * sizeof(double2);
+}
+
 template <int ME>
-static void launch_theta(tc_ctx *c, bool fwd, long long nrows, unsigned blocks, size_t lds) {
-  if (fwd)
+static void launch_theta(tc_ctx *c, bool fwd, long long nrows, unsigned blocks, size_t lds,
+                         const DivIn *dv) {
+  if (fwd && dv && c->divfft_stage)
+    hipLaunchKernelGGL((k_div_fft_theta_fwd<ME, true>), dim3(blocks), dim3(256), lds, c->stream,
+                       *dv, c->gd, c->spec, c->plan_t, c->tw_t, nrows, c->nth2, c->sstr);
+  else if (fwd && dv)
+    hipLaunchKernelGGL((k_div_fft_theta_fwd<ME, false>), dim3(blocks), dim3(256), lds, c->stream,
+                       *dv, c->gd, c->spec, c->plan_t, c->tw_t, nrows, c->nth2, c->sstr);
+  else if (fwd)
     hipLaunchKernelGGL(k_fft_theta_fwd<ME>, dim3(blocks), dim3(256), lds, c->stream, c->f[TC_DPH],
                        c->spec, c->plan_t, c->tw_t, nrows, c->nth2, c->sstr);
   else
@@ -390,24 +617,24 @@ static void launch_theta(tc_ctx *c, bool fwd, long long nrows, unsigned blocks, 
                        c->f[TC_DPH], c->plan_t, c->tw_t, nrows, c->nth2, c->sstr);
 }
 
-static void theta_pass(tc_ctx *c, bool fwd) {
+static void theta_pass(tc_ctx *c, bool fwd, const DivIn *dv = nullptr) {
   const long long nrows = (long long)c->nrm * c->nzm;
   const long long npairs = (nrows + 1) / 2;
   const FFTPlan &pt = c->plan_t;
-  const size_t lds = (size_t)pt.lines * pt.n * sizeof(double2);
+  const size_t lds = fft_lds_bytes(pt);
   const unsigned blocks = (unsigned)((npairs + pt.lines - 1) / pt.lines);
   switch (pt.me) {
-    case 4: launch_theta<4>(c, fwd, nrows, blocks, lds); break;
-    case 8: launch_theta<8>(c, fwd, nrows, blocks, lds); break;
-    case 16: launch_theta<16>(c, fwd, nrows, blocks, lds); break;
-    default: launch_theta<32>(c, fwd, nrows, blocks, lds); break;
+    case 4: launch_theta<4>(c, fwd, nrows, blocks, lds, dv); break;
+    case 8: launch_theta<8>(c, fwd, nrows, blocks, lds, dv); break;
+    case 16: launch_theta<16>(c, fwd, nrows, blocks, lds, dv); break;
+    default: launch_theta<32>(c, fwd, nrows, blocks, lds, dv); break;
   }
 }
 
 template <int NT>
 static void z_pass_nt(tc_ctx *c, double sg) {
   const FFTPlan &pz = c->plan_z;
-  const size_t lds = (size_t)pz.lines * pz.n * sizeof(double2);
+  const size_t lds = fft_lds_bytes(pz);
   const int ntile = (c->nth2 + pz.lines - 1) / pz.lines;
   const dim3 grid((unsigned)(ntile * c->nrm));
 #define TC_ZLAUNCH(ME)                                                                          \
@@ -429,17 +656,33 @@ static void z_pass(tc_ctx *c, double sg) {
     z_pass_nt<256>(c, sg);
 }
 
-void launch_poisson(tc_ctx *c) {
+void launch_poisson(tc_ctx *c, bool with_divergence, double usdt) {
   const long long M = (long long)c->nzm * c->sstr;
-  timer_begin(c, "fft_theta_fwd");
-  theta_pass(c, true);
+  if (with_divergence) {
+    const DivIn dv{c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], usdt};
+    timer_begin(c, "div_fft_theta_fwd");
+    theta_pass(c, true, &dv);
+  } else {
+    timer_begin(c, "fft_theta_fwd");
+    theta_pass(c, true);
+  }
   timer_end(c);
   timer_begin(c, "fft_z_fwd");
   z_pass(c, -1.0);
   timer_end(c);
   timer_begin(c, "poisson_tridiag");
-  hipLaunchKernelGGL(k_poisson_tridiag, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
-                     c->gd, c->spec, c->cpw, 1.0 / ((double)c->nthm * (double)c->nzm));
+  {
+    int pt, lb;
+    const int pq = c->poisson_pq;
+    poisson_part_shape(c->nrm, pq, pt, lb);
+    const size_t lds = sizeof(double) * (4 * 2 * (size_t)pt * lb + 3 * (size_t)c->nrm);
+    const dim3 grid((unsigned)((M + lb - 1) / lb)), block(lb, pt);
+    const double inv_norm = 1.0 / ((double)c->nthm * (double)c->nzm);
+    if (pq == 13)
+      hipLaunchKernelGGL(k_poisson_tridiag<13>, grid, block, lds, c->stream, c->gd, c->spec, inv_norm);
+    else
+      hipLaunchKernelGGL(k_poisson_tridiag<26>, grid, block, lds, c->stream, c->gd, c->spec, inv_norm);
+  }
   timer_end(c);
   timer_begin(c, "fft_z_inv");
   z_pass(c, 1.0);

@@ -1,0 +1,96 @@
+"""Parity at the production shape, 512 x 512 x 385 (theta x z x r), through
+properties that do not need the oracle (which takes minutes per step there):
+
+- the fused kernels and the routine-by-routine kernels (two independent
+  implementations of the same step) agree to 1e-10 relative L2 after two
+  RK3 steps, and each step's maximum divergence stays below RESID;
+- an axisymmetric, z-independent Couette start stays axisymmetric and
+  z-independent (every r-plane of qt uniform) with q_r and q_z at roundoff;
+- the projection is idempotent: projecting a projected field changes it by
+  roundoff only, and leaves the divergence at roundoff.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _production(tc):
+    return tc.configs.production_512()
+
+
+def test_fused_and_routine_paths_agree_full_size(tc):
+    p = _production(tc)
+    print("full size: fused vs routine-by-routine, 2 steps", flush=True)
+    qt, qr, qz = tc.init_perturbed_couette(p, tc.configs.SEEDS["production_512"], 1e-2)
+    runs = []
+    for fused in (True, False):
+        dev = tc.Solver(p)
+        dev.set_fused(fused)
+        dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+        dev.project()
+        for n in range(2):
+            code, d = dev.step(n)
+            assert code == 0, (fused, n, code)
+            assert d["dmax"] < p["resid"], (fused, n, d)
+        runs.append({f: dev.get(f) for f in ("qt", "qr", "qz", "pr")})
+        dev.close()
+    del qt, qr, qz
+    nrm = p["nrm"]
+    for f in ("qt", "qr", "qz", "pr"):
+        sl = slice(0, nrm + 1) if f == "qr" else slice(0, nrm)
+        e = rel_l2(runs[0][f][sl], runs[1][f][sl])
+        print(f"  {f}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-10, (f, e)
+
+
+def test_axisymmetric_couette_stays_axisymmetric_full_size(tc):
+    p = _production(tc)
+    print("full size: axisymmetric Couette start, 2 steps", flush=True)
+    dev = tc.Solver(p)
+    nrm = p["nrm"]
+    rm = tc.grid_vector(p, "rm")[:nrm]
+    eta = p["rint"] / p["rext"]
+    A = (p["virotext"] - eta * eta * p["virotint"]) / (1 - eta * eta)
+    B = (p["virotint"] - p["virotext"]) * p["rint"] ** 2 / (1 - eta * eta)
+    shape = (nrm + 1, p["nzm"], p["nthm"])
+    qt = np.zeros(shape, order="F")
+    qt[:nrm] = (A * rm + B / rm)[:, None, None]
+    dev.set("qt", qt)
+    del qt
+    assert dev.CheckDivergence() == 0.0
+    for n in range(2):
+        code, d = dev.step(n)
+        assert code == 0
+    qt, qr, qz = dev.get("qt")[:nrm], dev.get("qr"), dev.get("qz")[:nrm]
+    scale = np.abs(qt).max()
+    spread = (qt.max(axis=(1, 2)) - qt.min(axis=(1, 2))).max()
+    print(f"  qt plane spread {spread:.2e}, max|qr| {np.abs(qr).max():.2e}, "
+          f"max|qz| {np.abs(qz).max():.2e}, scale {scale:.3f}", flush=True)
+    assert spread <= 1e-13 * scale
+    assert np.abs(qr).max() <= 1e-12 * scale
+    assert np.abs(qz).max() <= 1e-12 * scale
+    dev.close()
+
+
+def test_projection_idempotent_full_size(tc):
+    p = _production(tc)
+    print("full size: projection idempotence", flush=True)
+    dev = tc.Solver(p)
+    qt, qr, qz = tc.init_perturbed_couette(p, 99, 1e-2)
+    dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+    del qt, qr, qz
+    assert dev.CheckDivergence() > 1e-3
+    dev.project()
+    d1 = dev.CheckDivergence()
+    a = {f: dev.get(f) for f in ("qt", "qr", "qz")}
+    dev.project()
+    d2 = dev.CheckDivergence()
+    print(f"  max div after projections: {d1:.2e}, {d2:.2e}", flush=True)
+    assert d1 < 1e-11 and d2 < 1e-11
+    for f in ("qt", "qr", "qz"):
+        e = np.abs(dev.get(f) - a[f]).max() / np.abs(a["qt"]).max()
+        assert e <= 1e-13, (f, e)
+    dev.close()

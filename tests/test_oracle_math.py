@@ -284,3 +284,87 @@ def test_main_loop_error_codes(tc, oracle):
     assert _step_codes(oracle, dict(base, idtv=0, dtmax=1e-3, limitcfl=1e-9), 5) == [165]
     assert _step_codes(oracle, dict(base, idtv=1, dtmin=2e-3, dtmax=1e-2, limitcfl=1e-9), 5) == [0, 0, 166]
     assert _step_codes(oracle, dict(base, walltimemax=1.0), 5, walltime=2.0) == [334]
+
+
+def _reduction_oracle(tc, oracle, nthm=12, nzm=10, nrm=9):
+    p = params(tc, nthm, nzm, nrm, alx3=1.7, lamb=3.0)
+    return p, oracle.Oracle(p)
+
+
+def test_max_cfl_closed_form(tc, oracle):
+    """CalcMaxCFL.F90:22-39.  Solid body qt = Omega rm gives the same
+    azimuthal term Omega dxt at every radius; a uniform qz adds W dxz; the
+    zero field returns the 1e-8 floor."""
+    p, o = _reduction_oracle(tc, oracle)
+    shape = o.shape
+    assert o.max_cfl() == 1e-8
+    rm = o.vec("rm")
+    Om, W = 0.8, -0.3
+    qt = np.zeros(shape, order="F")
+    qt[: o.nrm] = Om * rm[: o.nrm][:, None, None]
+    qz = np.full(shape, W, order="F")
+    o.set("qt", qt)
+    o.set("qz", qz)
+    dxt = p["nthm"] * p["lamb"] / (2 * np.pi)
+    dxz = p["nzm"] / p["alx3"]
+    assert abs(o.max_cfl() - (Om * dxt + abs(W) * dxz)) <= 1e-13 * (Om * dxt + abs(W) * dxz)
+    o.close()
+
+
+def test_check_divergence_closed_forms(tc, oracle):
+    """CheckDivergence.F90:21-40.  qt = f(z), qz = g(theta), qr = 0 is
+    discretely solenoidal (exact zero); a one-point spike in qz gives
+    max |div| = dxz; a spike in qt at theta index 0 on every (r, z) gives
+    dxt / rm(1); a spike qr(2) = 1 gives udx2m(k)/rm(k) at k = 1, 2."""
+    p, o = _reduction_oracle(tc, oracle)
+    rng = np.random.default_rng(3)
+    shape = o.shape
+    dxt = p["nthm"] * p["lamb"] / (2 * np.pi)
+    dxz = p["nzm"] / p["alx3"]
+    qt = np.zeros(shape, order="F")
+    qt[: o.nrm] = rng.uniform(-1, 1, o.nzm)[None, :, None]
+    qz = np.zeros(shape, order="F")
+    qz[: o.nrm] = rng.uniform(-1, 1, o.nthm)[None, None, :]
+    o.set("qt", qt)
+    o.set("qz", qz)
+    o.set("qr", np.zeros(shape))
+    assert o.max_div() == 0.0
+    o.set("qt", np.zeros(shape))
+    qz = np.zeros(shape, order="F")
+    qz[3, 4, 5] = 1.0
+    o.set("qz", qz)
+    assert abs(o.max_div() - dxz) <= 1e-14 * dxz
+    o.set("qz", np.zeros(shape))
+    qt = np.zeros(shape, order="F")
+    qt[: o.nrm, :, 0] = 1.0
+    o.set("qt", qt)
+    rm = o.vec("rm")
+    assert abs(o.max_div() - dxt / rm[0]) <= 1e-14 * dxt / rm[0]
+    o.set("qt", np.zeros(shape))
+    qr = np.zeros(shape, order="F")
+    qr[1] = 1.0
+    o.set("qr", qr)
+    rc = o.vec("rc")
+    # udx2m(k) = dxr / g2rm(k) with g2rm(k) = (rc(k+1) - rc(k)) dxr: 1/(rc(k+1)-rc(k))
+    want = max(1.0 / ((rc[1] - rc[0]) * rm[0]), 1.0 / ((rc[2] - rc[1]) * rm[1]))
+    assert abs(o.max_div() - want) <= 1e-12 * want
+    o.close()
+
+
+def test_velocity_stats_closed_forms(tc, oracle):
+    """CalcVelocityStats.F90:29-60.  Uniform qt = U, qz = W and qr = V rc
+    (so that q_r / r = V) give mean squares U^2, V^2, W^2 exactly, because the
+    pvol(k) = rc(k+1)^2 - rc(k)^2 weights telescope to rext^2 - rint^2; vmax
+    is the largest magnitude over kc = 1..nrm."""
+    p, o = _reduction_oracle(tc, oracle)
+    shape = o.shape
+    U, V, W = 1.3, -0.4, 0.7
+    rc = o.vec("rc")
+    o.set("qt", np.full(shape, U))
+    o.set("qr", np.broadcast_to(V * rc[: o.nr][:, None, None], shape))
+    o.set("qz", np.full(shape, W))
+    vmax, rms = o.velocity_stats()
+    assert np.allclose(rms, [U * U, V * V, W * W], rtol=1e-13, atol=0)
+    assert vmax[0] == U and vmax[2] == W
+    assert vmax[1] == abs(V) * max(abs(rc[: o.nrm]))
+    o.close()

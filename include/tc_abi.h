@@ -77,6 +77,30 @@ enum {
  * Allocates the device state; all fields start at zero
  * (AuxiliaryRoutines.F90:123). */
 int tc_create(const tc_params *p, tc_ctx **out);
+
+/* Kernel schedule knobs of one context (no counterpart in the reference,
+ * whose schedule is fixed by its MPI decomposition and FFTW plans).  Zero
+ * fields take the TC_* environment override, else the default. */
+typedef struct tc_tuning {
+  int helm_lpt;   /* radial lines per thread of the fused Helmholtz kernel: 1, 2 (0: by occupancy) */
+  int fft_t_lds;  /* LDS bytes of a theta-FFT block (default 32768) */
+  int fft_z_lds;  /* LDS bytes of a z-FFT block (default 65536) */
+  int fft_z_nt;   /* threads of a z-FFT block: 256 or 512 (default) */
+  int fft_pad;    /* LDS padding of the FFT tiles: 1 on (default), -1 off */
+  int poisson_pq; /* rows per thread of the radial Poisson solve: 13 or 26 (default) */
+} tc_tuning;
+
+/* tc_create with an explicit schedule; tc_set_tuning re-plans an existing
+ * context (fields untouched); tc_get_tuning returns the effective values. */
+int tc_create_tuned(const tc_params *p, const tc_tuning *t, tc_ctx **out);
+int tc_set_tuning(tc_ctx *ctx, const tc_tuning *t);
+int tc_get_tuning(tc_ctx *ctx, tc_tuning *t);
+
+/* Autotuner (no counterpart in the reference): times `steps` fused RK3
+ * steps of this grid shape per candidate schedule (coordinate descent over
+ * the knobs) and returns the fastest.  Results are cached per shape for the
+ * life of the process. */
+int tc_autotune(const tc_params *p, int steps, tc_tuning *best);
 int tc_destroy(tc_ctx *ctx);
 const char *tc_last_error(void);
 
@@ -169,6 +193,42 @@ int tc_copy_field(tc_ctx *ctx, int dst, int src);
 
 /* Device bytes held by the context (fields + scratch). */
 int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
+
+/* ---------------- batched sweep (no counterpart in the reference) ---------------- */
+
+/* Many independent instances, each a tc_ctx of its own shape and
+ * parameters, advanced together: instances of one grid shape (and schedule)
+ * form a group, and every kernel of the fused RK3 step runs as ONE grouped
+ * launch per group (blockIdx selects the instance through a device table of
+ * per-instance descriptors).  A step of all groups can be captured once as a
+ * hipGraph and replayed.  Instances advance at a fixed dt (IDTV=0, dt =
+ * DTMAX) with the main.F90 loop semantics: CFL guard 165 every step,
+ * logging-step diagnostics with 169/266, 333 past TMAX, 555 after NTST.
+ * Device-detected codes are read back every `sync_every` steps; the report
+ * names the first failing step, as the reference would exit there. */
+typedef struct tc_batch tc_batch;
+
+int tc_batch_create(tc_batch **out);
+int tc_batch_destroy(tc_batch *b);
+/* Adds an instance (tc_create_tuned); t may be NULL.  *id identifies it. */
+int tc_batch_add(tc_batch *b, const tc_params *p, const tc_tuning *t, int *id);
+/* Removes an instance and frees its device memory. */
+int tc_batch_remove(tc_batch *b, int id);
+/* The instance's context, for uploads, downloads, projection and the
+ * single-instance reductions; waits for the batch's queued work first.
+ * Work queued on the context itself is waited for by the next advance. */
+int tc_batch_ctx(tc_batch *b, int id, tc_ctx **ctx);
+/* Advances every unfinished instance by up to nsteps steps (each stops at
+ * its own end); use_graph: replay one captured hipGraph per step.  Returns
+ * the number of unfinished instances, or a negative error. */
+int tc_batch_advance(tc_batch *b, int nsteps, int use_graph, int sync_every);
+/* Progress of one instance: steps taken, error code (0 while running; the
+ * main.F90 codes once finished), and out9 = cfl dt time dmax torque_in
+ * torque_out vmax1 vmax2 vmax3 of its last logging step. */
+int tc_batch_status(tc_batch *b, int id, int *steps, int *code, double *out9);
+/* Number of shape groups and grouped launches per step of the current
+ * membership (for reporting). */
+int tc_batch_groups(tc_batch *b, int *ngroups, int *launches_per_step);
 
 #ifdef __cplusplus
 }

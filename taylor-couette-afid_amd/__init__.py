@@ -57,7 +57,12 @@ ABI_SYMBOLS = [
     "tc_calc_max_cfl", "tc_check_divergence", "tc_calc_torque", "tc_calc_velocity_stats",
     "tc_step", "tc_project", "tc_init_perturbed_couette", "tc_synchronize",
     "tc_kernel_timing", "tc_kernel_timing_read", "tc_copy_field", "tc_device_bytes",
+    "tc_create_tuned", "tc_set_tuning", "tc_get_tuning", "tc_autotune",
+    "tc_batch_create", "tc_batch_destroy", "tc_batch_add", "tc_batch_remove", "tc_batch_ctx",
+    "tc_batch_advance", "tc_batch_status", "tc_batch_groups",
 ]
+
+TUNING_FIELDS = ["helm_lpt", "fft_t_lds", "fft_z_lds", "fft_z_nt", "fft_pad", "poisson_pq"]
 
 
 class Params(ctypes.Structure):
@@ -69,6 +74,21 @@ def make_params(d):
     for name, _ in PARAM_FIELDS:
         setattr(p, name, d.get(name, 0))
     return p
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in TUNING_FIELDS]
+
+
+def make_tuning(d):
+    t = Tuning()
+    for n in TUNING_FIELDS:
+        setattr(t, n, int((d or {}).get(n, 0)))
+    return t
+
+
+def tuning_dict(t):
+    return {n: getattr(t, n) for n in TUNING_FIELDS}
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -120,6 +140,19 @@ def lib():
                                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
     L.tc_device_bytes.argtypes = [_V, ctypes.POINTER(ctypes.c_ulonglong)]
     L.tc_copy_field.argtypes = [_V, ctypes.c_int, ctypes.c_int]
+    _T = ctypes.POINTER(Tuning)
+    L.tc_create_tuned.argtypes = [ctypes.POINTER(Params), _T, ctypes.POINTER(_V)]
+    L.tc_set_tuning.argtypes = [_V, _T]
+    L.tc_get_tuning.argtypes = [_V, _T]
+    L.tc_autotune.argtypes = [ctypes.POINTER(Params), ctypes.c_int, _T]
+    L.tc_batch_create.argtypes = [ctypes.POINTER(_V)]
+    L.tc_batch_destroy.argtypes = [_V]
+    L.tc_batch_add.argtypes = [_V, ctypes.POINTER(Params), _T, _I]
+    L.tc_batch_remove.argtypes = [_V, ctypes.c_int]
+    L.tc_batch_ctx.argtypes = [_V, ctypes.c_int, ctypes.POINTER(_V)]
+    L.tc_batch_advance.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.tc_batch_status.argtypes = [_V, ctypes.c_int, _I, _I, _D]
+    L.tc_batch_groups.argtypes = [_V, _I, _I]
     _LIB = L
     return L
 
@@ -155,20 +188,38 @@ def init_perturbed_couette(params, seed, amp):
 class Solver:
     """One device-resident Taylor-Couette instance with the AFiD routine names."""
 
-    def __init__(self, params):
+    def __init__(self, params, tuning=None, _handle=None):
         self.params = dict(params)
         self._p = make_params(self.params)
-        h = _V()
-        _check(lib().tc_create(ctypes.byref(self._p), ctypes.byref(h)))
-        self.h = h
+        self._owned = _handle is None
+        if _handle is None:
+            h = _V()
+            if tuning is None:
+                _check(lib().tc_create(ctypes.byref(self._p), ctypes.byref(h)))
+            else:
+                t = make_tuning(tuning)
+                _check(lib().tc_create_tuned(ctypes.byref(self._p), ctypes.byref(t), ctypes.byref(h)))
+            self.h = h
+        else:
+            self.h = _handle
         self.nthm, self.nzm, self.nrm = params["nthm"], params["nzm"], params["nrm"]
         self.nr = self.nrm + 1
         self.shape = (self.nr, self.nzm, self.nthm)
 
     def close(self):
         if getattr(self, "h", None):
-            lib().tc_destroy(self.h)
+            if self._owned:
+                lib().tc_destroy(self.h)
             self.h = None
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().tc_get_tuning(self.h, ctypes.byref(t)))
+        return tuning_dict(t)
+
+    def set_tuning(self, tuning):
+        t = make_tuning(tuning)
+        _check(lib().tc_set_tuning(self.h, ctypes.byref(t)))
 
     def __del__(self):
         try:
@@ -304,3 +355,78 @@ class Solver:
         v = ctypes.c_ulonglong()
         _check(lib().tc_device_bytes(self.h, ctypes.byref(v)))
         return v.value
+
+
+def autotune(params, steps=3):
+    """Fastest schedule for this grid shape (timed on the device, cached per
+    shape for the life of the process)."""
+    p = make_params(params)
+    t = Tuning()
+    _check(lib().tc_autotune(ctypes.byref(p), int(steps), ctypes.byref(t)))
+    return tuning_dict(t)
+
+
+class Batch:
+    """Many independent instances advanced together with grouped launches
+    (one launch per kernel per shape group) and an optional hipGraph per
+    step.  Instances advance at a fixed dt (IDTV = 0, dt = DTMAX)."""
+
+    STATUS = ["cfl", "dt", "time", "dmax", "torque_in", "torque_out", "vmax1", "vmax2", "vmax3"]
+
+    def __init__(self):
+        h = _V()
+        _check(lib().tc_batch_create(ctypes.byref(h)))
+        self.h = h
+        self.params = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, params, tuning=None):
+        p = make_params(params)
+        i = ctypes.c_int()
+        if tuning is None:
+            _check(lib().tc_batch_add(self.h, ctypes.byref(p), None, ctypes.byref(i)))
+        else:
+            t = make_tuning(tuning)
+            _check(lib().tc_batch_add(self.h, ctypes.byref(p), ctypes.byref(t), ctypes.byref(i)))
+        self.params[i.value] = dict(params)
+        return i.value
+
+    def remove(self, iid):
+        _check(lib().tc_batch_remove(self.h, int(iid)))
+        self.params.pop(iid, None)
+
+    def solver(self, iid):
+        """Solver view of one instance (not owned: the batch frees it)."""
+        h = _V()
+        _check(lib().tc_batch_ctx(self.h, int(iid), ctypes.byref(h)))
+        return Solver(self.params[iid], _handle=h)
+
+    def advance(self, nsteps, graph=True, sync_every=16):
+        return _check(lib().tc_batch_advance(self.h, int(nsteps), int(bool(graph)), int(sync_every)))
+
+    def status(self, iid):
+        steps, code = ctypes.c_int(), ctypes.c_int()
+        o = np.zeros(9)
+        _check(lib().tc_batch_status(self.h, int(iid), ctypes.byref(steps), ctypes.byref(code), _dp(o)))
+        return steps.value, code.value, dict(zip(self.STATUS, o))
+
+    def groups(self):
+        n, l = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tc_batch_groups(self.h, ctypes.byref(n), ctypes.byref(l)))
+        return n.value, l.value

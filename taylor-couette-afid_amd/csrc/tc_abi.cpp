@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <tuple>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -267,6 +269,39 @@ void set_substep(tc_ctx *c, int ns) {
 }
 
 
+// Effective tuning: explicit nonzero fields first, then the TC_* environment
+// overrides, then the defaults (which come from sweeps at 512x512x385).
+tc_tuning resolve_tuning(const tc_tuning *in) {
+  tc_tuning t{};
+  if (in) t = *in;
+  auto env = [](const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+  };
+  if (t.helm_lpt == 0) t.helm_lpt = std::max(0, std::min(2, env("TC_HELM_LPT", 0)));
+  if (t.fft_t_lds == 0) t.fft_t_lds = env("TC_FFT_T_LDS", 32768);
+  if (t.fft_z_lds == 0) t.fft_z_lds = env("TC_FFT_Z_LDS", 65536);
+  if (t.fft_z_nt == 0) t.fft_z_nt = env("TC_FFT_Z_NT", 512) == 256 ? 256 : 512;
+  if (t.fft_pad == 0) t.fft_pad = env("TC_FFT_PAD", 1) == 0 ? -1 : 1;
+  if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", 26) == 13 ? 13 : 26;
+  return t;
+}
+
+// theta: row-pair FFTs within fft_t_lds bytes of LDS per block; z: column
+// tiles within fft_z_lds, as wide as that allows (wider tiles = longer row
+// segments)
+std::string apply_plans(tc_ctx *c, const tc_tuning &t) {
+  tc::FFTPlan pt, pz;
+  std::string e = tc::make_fft_plan(c->p.nthm, pt, t.fft_t_lds, 256);
+  if (e.empty()) e = tc::make_fft_plan(c->p.nzm, pz, t.fft_z_lds, t.fft_z_nt == 256 ? 256 : 512);
+  if (!e.empty()) return e;
+  if (t.fft_pad < 0) pt.padsh = pz.padsh = 31;
+  c->plan_t = pt;
+  c->plan_z = pz;
+  c->tuning = t;
+  return std::string();
+}
+
 // CalcTorque.F90:26-55: normalisation of the masked wall sums
 void torque_from_sums(const tc_ctx *c, double slow, double sup, double *inner, double *outer) {
   const tc_params &p = c->p;
@@ -285,6 +320,14 @@ void torque_from_sums(const tc_ctx *c, double slow, double sup, double *inner, d
 }  // namespace
 
 namespace tc {
+
+int ctx_begin_step(tc_ctx *c) { return begin_step(c); }
+void ctx_set_substep(tc_ctx *c, int ns) { set_substep(c, ns); }
+void ctx_torque(const tc_ctx *c, double slow, double sup, double *inner, double *outer) {
+  torque_from_sums(c, slow, sup, inner, outer);
+}
+int set_error(const std::string &msg) { return fail(msg); }
+int check_launch() { return launch_status(); }
 
 void timer_begin(tc_ctx *c, const char *name) {
   if (!c->timing) return;
@@ -325,26 +368,17 @@ int tc_grid_vector(const tc_params *p, int id, double *out, int n) {
   return m;
 }
 
-int tc_create(const tc_params *p, tc_ctx **out) {
+int tc_create(const tc_params *p, tc_ctx **out) { return tc_create_tuned(p, nullptr, out); }
+
+int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   if (!p || !out) return fail("null argument");
   *out = nullptr;
   tc_ctx *c = new (std::nothrow) tc_ctx();
   if (!c) return fail("out of host memory");
   c->p = *p;
   std::string e = tc::build_grid(*p, c->g);
-  // theta: row-pair FFTs, up to 32 KiB of LDS per block; z: column tiles,
-  // up to 64 KiB, as wide as that allows (wider tiles = longer row segments)
-  {
-    // LDS budgets (bytes) of the FFT tiles; TC_FFT_T_LDS / TC_FFT_Z_LDS override
-    const char *et = std::getenv("TC_FFT_T_LDS"), *ez = std::getenv("TC_FFT_Z_LDS");
-    const char *ezt = std::getenv("TC_FFT_Z_NT");
-    const int zt = (ezt && std::atoi(ezt) == 256) ? 256 : 512;
-    const int lt = et ? std::atoi(et) : 32768, lz = ez ? std::atoi(ez) : 65536;
-    if (e.empty()) e = tc::make_fft_plan(p->nthm, c->plan_t, lt, 256);
-    if (e.empty()) e = tc::make_fft_plan(p->nzm, c->plan_z, lz, zt);
-    const char *epad = std::getenv("TC_FFT_PAD");
-    if (epad && std::atoi(epad) == 0) c->plan_t.padsh = c->plan_z.padsh = 31;
-  }
+  const tc_tuning t = resolve_tuning(tun);
+  if (e.empty()) e = apply_plans(c, t);
   if (!e.empty()) {
     delete c;
     return fail(e);
@@ -460,13 +494,9 @@ int tc_create(const tc_params *p, tc_ctx **out) {
   TRY(dev_alloc(c, (void **)&c->red_out, sizeof(double) * 16));
   HTRY(hipHostMalloc((void **)&c->red_host, sizeof(double) * 16, hipHostMallocDefault));
   set_substep(c, 1);
+  c->helm_lpt = t.helm_lpt ? t.helm_lpt : tc::choose_helm_lpt(c);
+  c->poisson_pq = t.poisson_pq;
   {
-    // lines per thread of the fused Helmholtz kernel (TC_HELM_LPT overrides)
-    const char *lp = std::getenv("TC_HELM_LPT");
-    c->helm_lpt = lp ? std::max(1, std::min(2, std::atoi(lp))) : tc::choose_helm_lpt(c);
-    // rows per thread of the radial Poisson solve (TC_POISSON_PQ overrides)
-    const char *pq = std::getenv("TC_POISSON_PQ");
-    c->poisson_pq = (pq && std::atoi(pq) == 13) ? 13 : 26;
     const char *dfs = std::getenv("TC_DIVFFT_STAGE");
     c->divfft_stage = dfs ? std::atoi(dfs) : 0;
   }
@@ -475,6 +505,116 @@ int tc_create(const tc_params *p, tc_ctx **out) {
 #undef TRY
 #undef HTRY
   *out = c;
+  return 0;
+}
+
+int tc_set_tuning(tc_ctx *c, const tc_tuning *t) {
+  CTXCHK(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const tc_tuning r = resolve_tuning(t);
+  const std::string e = apply_plans(c, r);
+  if (!e.empty()) return fail(e);
+  c->helm_lpt = r.helm_lpt ? r.helm_lpt : tc::choose_helm_lpt(c);
+  c->poisson_pq = r.poisson_pq;
+  return 0;
+}
+
+int tc_get_tuning(tc_ctx *c, tc_tuning *t) {
+  CTXCHK(c);
+  if (!t) return fail("null argument");
+  *t = c->tuning;
+  t->helm_lpt = c->helm_lpt;
+  t->poisson_pq = c->poisson_pq;
+  return 0;
+}
+
+// Coordinate descent over the schedule knobs on one context of the shape:
+// each candidate is timed over `steps` fused RK3 steps (after one untimed
+// step) and kept if it is at least 2% faster.  The fields stay zero, which
+// exercises the same instruction stream as any other state.
+int tc_autotune(const tc_params *p, int steps, tc_tuning *best) {
+  if (!p || !best) return fail("null argument");
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, tc_tuning> cache;
+  const auto key = std::make_tuple(p->nthm, p->nzm, p->nrm);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      *best = it->second;
+      return 0;
+    }
+  }
+  if (steps < 1) steps = 1;
+  tc_params q = *p;
+  q.idtv = 0;
+  if (!(q.dtmax > 0.0)) q.dtmax = 1e-4;
+  tc_ctx *c = nullptr;
+  int rc = tc_create(&q, &c);
+  if (rc) return rc;
+  c->dt = q.dtmax;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    tc_destroy(c);
+    return fail("tc_autotune: event");
+  }
+  auto run = [&](const tc_tuning &t, double &ms) -> int {
+    int r = tc_set_tuning(c, &t);
+    if (r) return r;
+    r = tc_time_marcher(c);
+    if (r) return r;
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int s = 0; s < steps; s++) {
+      r = tc_time_marcher(c);
+      if (r) return r;
+    }
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, e0, e1));
+    ms = f;
+    return 0;
+  };
+  tc_tuning cur{};
+  tc_get_tuning(c, &cur);
+  double best_ms = 0.0;
+  rc = run(cur, best_ms);
+  struct Knob {
+    int tc_tuning::*field;
+    std::vector<int> values;
+  };
+  const Knob knobs[] = {{&tc_tuning::helm_lpt, {1, 2}},
+                        {&tc_tuning::poisson_pq, {26, 13}},
+                        {&tc_tuning::fft_t_lds, {16384, 32768, 65536}},
+                        {&tc_tuning::fft_z_lds, {32768, 65536}},
+                        {&tc_tuning::fft_z_nt, {256, 512}},
+                        {&tc_tuning::fft_pad, {1, -1}}};
+  for (const Knob &k : knobs) {
+    if (rc) break;
+    for (int v : k.values) {
+      if (cur.*k.field == v) continue;
+      tc_tuning cand = cur;
+      cand.*k.field = v;
+      double ms = 0.0;
+      if (run(cand, ms) != 0) {
+        hipGetLastError();  // a schedule that does not fit is skipped
+        continue;
+      }
+      if (ms < 0.98 * best_ms) {
+        best_ms = ms;
+        cur = cand;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  tc_destroy(c);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = cur;
+  }
+  *best = cur;
   return 0;
 }
 

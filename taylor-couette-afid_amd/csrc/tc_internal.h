@@ -45,6 +45,51 @@ struct Timer {
   hipEvent_t a, b;
 };
 
+// Scalars of one substep's implicit solves
+struct ImpArgs {
+  double al, ga, ro, dt, alre;
+  double vwall_in;  // virotint + vinner_t
+  double virotext;
+  double bc1;       // 0.5*al*dt*dvinner_t*am1j(1)/ren (SolveImpEqnUpdate_QT.F90:67)
+  double udxz;      // dxz*al (ImplicitAndUpdateQZ.F90:25)
+};
+
+// Everything a grouped launch needs about one instance for one substep; one
+// table of these per (shape group, substep) lives in device memory.
+struct StepDesc {
+  GridDev g;
+  ImpArgs a;                    // vwall_in and bc1 are overridden from BatchDyn
+  double *f[TC_NFIELDS];
+  double2 *spec;
+  const unsigned char *icyl, *ocyl;
+  const double *lut, *lur, *luz;  // LU tables of this substep
+  int lu_nt;
+  double dt, al, beta, usdt, inv_norm;
+  FFTPlan pt, pz;
+  const double2 *twt, *twz;
+  double *red, *red_out;        // reduction partials and results
+  double limitcfl, resid, limitvel;
+  double virotint, virotext, del, deln, pi;
+};
+
+// Per-instance scalars that change every step (written by the host before
+// each step): the inner-wall terms of the three substeps, whether this step
+// logs, and the step index.
+struct BatchDyn {
+  double vwall[3], bc1[3];
+  int log, ntime;
+};
+
+// Per-instance results kept on the device between host synchronisations.
+struct BatchRes {
+  double cfl;          // CFL at the start of the last step
+  double diag[10];     // last logging pass: cfl, max|div|, vmax(3), rms sums(3), torque sums(2)
+  int fail165;         // 1 + ntime of the first step with cfl*dt > CFLMAX (0: none)
+  int log_ntime;       // ntime of the last logging pass (-1: none)
+  int faillog;         // 1 + ntime of the first logging step with code 169 or 266 (0: none)
+  int faillog_code;    // that code
+};
+
 }  // namespace tc
 
 struct tc_ctx {
@@ -83,6 +128,7 @@ struct tc_ctx {
   int divfft_stage = 0;  // fused divergence FFT: stage the divergence rows in LDS first
   int poisson_pq = 26;  // rows per thread of the radial Poisson solve (13 or 26)
   int helm_lpt = 1;  // radial lines per thread in the fused Helmholtz kernel (1 or 2)
+  tc_tuning tuning{};  // effective tuning (helm_lpt 0 = chosen by occupancy)
   // CFL of the current velocity, when known (set by the logging pass of tc_step)
   bool cfl_valid = false;
   double cfl_cached = 0.0;
@@ -119,12 +165,31 @@ void launch_velocity_stats(tc_ctx *c);
 void launch_diagnostics(tc_ctx *c);
 void launch_copy(tc_ctx *c, int dst, int src);
 
+// grouped launches over n instances of one shape (c: any instance of the
+// group, for the shape and the tuning); tab: device table of n StepDesc of
+// the substep, dyn/res: device arrays of n
+void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
+                              hipStream_t s);
+void launch_correct_both_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t s);
+void launch_cfl_check_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, BatchRes *res, int n,
+                        hipStream_t s);
+void launch_diagnostics_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, BatchRes *res, int n,
+                          hipStream_t s);
+void launch_poisson_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t s);
+
 // Poisson solve (tc_poisson.hip)
 std::string make_fft_plan(int n, FFTPlan &pl, int lds_bytes, int nt);
 void fill_twiddles(int n, std::vector<double> &tw);
 // with_divergence: compute dph = div(q)*usdt inside the forward theta FFT
 // instead of reading it (the fused time marcher)
 void launch_poisson(tc_ctx *c, bool with_divergence = false, double usdt = 0.0);
+
+// context helpers shared with the batch layer (tc_abi.cpp)
+int ctx_begin_step(tc_ctx *c);
+void ctx_set_substep(tc_ctx *c, int ns);
+void ctx_torque(const tc_ctx *c, double slow, double sup, double *inner, double *outer);
+int set_error(const std::string &msg);
+int check_launch();
 
 // timing helpers (tc_abi.cpp)
 void timer_begin(tc_ctx *c, const char *name);

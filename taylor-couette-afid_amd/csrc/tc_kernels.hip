@@ -214,14 +214,6 @@ DEVI void lsolve_step(const LUView &T, int r, double f, double &carry, double &o
   }
 }
 
-struct ImpArgs {
-  double al, ga, ro, dt, alre;
-  double vwall_in;  // virotint + vinner_t
-  double virotext;
-  double bc1;       // 0.5*al*dt*dvinner_t*am1j(1)/ren (SolveImpEqnUpdate_QT.F90:67)
-  double udxz;      // dxz*al (ImplicitAndUpdateQZ.F90:25)
-};
-
 // Right-hand side row k of the q_theta solve (ImplicitAndUpdateQT.F90:45-115);
 // qm, q0, qp = qt(k-1), qt(k), qt(k+1) on the line.
 DEVI double rhs_qt(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
@@ -424,6 +416,26 @@ DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
 }
 
 // which: bit 0 q_theta, bit 1 q_r, bit 2 q_z
+DEVI void helm_bwd_body(const long long l, const GridDev &g, double *__restrict__ qt,
+                        double *__restrict__ qr, double *__restrict__ qz,
+                        const double *__restrict__ yt, const double *__restrict__ yr,
+                        const double *__restrict__ yz, const unsigned char *__restrict__ icyl,
+                        const unsigned char *__restrict__ ocyl, const double *__restrict__ lut,
+                        const double *__restrict__ lur, const double *__restrict__ luz, int nt,
+                        int which) {
+  if (l >= g.P) return;
+  const int v = icyl[l] + 2 * ocyl[l];
+  if (which & 1) usolve_line(lu_view(lut + (size_t)v * nt, nt), g.nrm, 0, qt, yt, l, g.P);
+  if (which & 2) {
+    // SolveImpEqnUpdate_QR.F90:63-65 adds rows 2..nrm; walls reset
+    // (ImplicitAndUpdateQR.F90:70-71)
+    usolve_line(lu_view(lur, nt), g.nr, 1, qr, yr, l, g.P);
+    qr[l] = 0.0;
+    qr[(long long)(g.nr - 1) * g.P + l] = 0.0;
+  }
+  if (which & 4) usolve_line(lu_view(luz + (size_t)v * nt, nt), g.nrm, 0, qz, yz, l, g.P);
+}
+
 __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict__ qt,
                                                   double *__restrict__ qr,
                                                   double *__restrict__ qz,
@@ -436,18 +448,16 @@ __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict_
                                                   const double *__restrict__ lur,
                                                   const double *__restrict__ luz, int nt,
                                                   int which) {
-  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= g.P) return;
-  const int v = icyl[l] + 2 * ocyl[l];
-  if (which & 1) usolve_line(lu_view(lut + (size_t)v * nt, nt), g.nrm, 0, qt, yt, l, g.P);
-  if (which & 2) {
-    // SolveImpEqnUpdate_QR.F90:63-65 adds rows 2..nrm; walls reset
-    // (ImplicitAndUpdateQR.F90:70-71)
-    usolve_line(lu_view(lur, nt), g.nr, 1, qr, yr, l, g.P);
-    qr[l] = 0.0;
-    qr[(long long)(g.nr - 1) * g.P + l] = 0.0;
-  }
-  if (which & 4) usolve_line(lu_view(luz + (size_t)v * nt, nt), g.nrm, 0, qz, yz, l, g.P);
+  helm_bwd_body((long long)blockIdx.x * blockDim.x + threadIdx.x, g, qt, qr, qz, yt, yr, yz, icyl,
+                ocyl, lut, lur, luz, nt, which);
+}
+
+// Grouped fused backward sweep: blockIdx.y = instance; y = rhs, dq, qcap.
+__global__ __launch_bounds__(256) void k_helm_bwd_g(const StepDesc *__restrict__ tab) {
+  const StepDesc &d = tab[blockIdx.y];
+  helm_bwd_body((long long)blockIdx.x * blockDim.x + threadIdx.x, d.g, d.f[TC_QT], d.f[TC_QR],
+                d.f[TC_QZ], d.f[TC_RHS], d.f[TC_DQ], d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur,
+                d.luz, d.lu_nt, 7);
 }
 
 // ---------------------------------------------------------------------------
@@ -483,17 +493,17 @@ DEVI int xcd_tile(int b, int nb) {
 // (no-slip everywhere, the SlipBCs default), so all lines share one table
 // entry and the table pointers stay scalar.
 template <bool UNIFORM, int LPT>
-__global__ __launch_bounds__(256) void k_helm_fwd(
-    GridDev g, ImpArgs a, const double *__restrict__ qt, const double *__restrict__ qr,
-    const double *__restrict__ qz, const double *__restrict__ pr, double *__restrict__ ru1,
-    double *__restrict__ ru2, double *__restrict__ ru3, double *__restrict__ yt,
-    double *__restrict__ yr, double *__restrict__ yz, const unsigned char *__restrict__ icyl,
-    const unsigned char *__restrict__ ocyl, const double *__restrict__ lut,
-    const double *__restrict__ lur, const double *__restrict__ luz, int nt, int ntx, int ntz) {
+DEVI void helm_fwd_body(
+    const int tile, const GridDev &g, const ImpArgs &a, const double *__restrict__ qt,
+    const double *__restrict__ qr, const double *__restrict__ qz, const double *__restrict__ pr,
+    double *__restrict__ ru1, double *__restrict__ ru2, double *__restrict__ ru3,
+    double *__restrict__ yt, double *__restrict__ yr, double *__restrict__ yz,
+    const unsigned char *__restrict__ icyl, const unsigned char *__restrict__ ocyl,
+    const double *__restrict__ lut, const double *__restrict__ lur,
+    const double *__restrict__ luz, int nt, int ntx) {
   constexpr int TJ = 4 * LPT, H = TJ + 2, S = FT_W * H, NE = (S + 255) / 256;
   __shared__ double sq[3][3][S];  // field, ring slot, tile
   __shared__ double sp[2][S];     // pressure planes k and k+1
-  const int tile = xcd_tile(blockIdx.x, ntx * ntz);
   const int i0 = (tile % ntx) * FT_I, j0 = (tile / ntx) * TJ;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const long long P = g.P;
@@ -655,6 +665,36 @@ __global__ __launch_bounds__(256) void k_helm_fwd(
     yr[(long long)(g.nr - 2) * P + l[u]] = yv;
     yr[(long long)(g.nr - 1) * P + l[u]] = c2;
   }
+}
+
+template <bool UNIFORM, int LPT>
+__global__ __launch_bounds__(256) void k_helm_fwd(
+    GridDev g, ImpArgs a, const double *__restrict__ qt, const double *__restrict__ qr,
+    const double *__restrict__ qz, const double *__restrict__ pr, double *__restrict__ ru1,
+    double *__restrict__ ru2, double *__restrict__ ru3, double *__restrict__ yt,
+    double *__restrict__ yr, double *__restrict__ yz, const unsigned char *__restrict__ icyl,
+    const unsigned char *__restrict__ ocyl, const double *__restrict__ lut,
+    const double *__restrict__ lur, const double *__restrict__ luz, int nt, int ntx, int ntz) {
+  helm_fwd_body<UNIFORM, LPT>(xcd_tile(blockIdx.x, ntx * ntz), g, a, qt, qr, qz, pr, ru1, ru2,
+                              ru3, yt, yr, yz, icyl, ocyl, lut, lur, luz, nt, ntx);
+}
+
+// Grouped: blockIdx.x = tile, blockIdx.y = instance of the shape group; the
+// XCD remap runs over the flattened (instance, tile) index.
+template <bool UNIFORM, int LPT>
+__global__ __launch_bounds__(256) void k_helm_fwd_g(const StepDesc *__restrict__ tab,
+                                                    const BatchDyn *__restrict__ dyn, int ns,
+                                                    int ntx, int ntz) {
+  const int ntiles = ntx * ntz;
+  const int lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int inst = lin / ntiles, tile = lin - inst * ntiles;
+  const StepDesc &d = tab[inst];
+  ImpArgs a = d.a;
+  a.vwall_in = dyn[inst].vwall[ns - 1];
+  a.bc1 = dyn[inst].bc1[ns - 1];
+  helm_fwd_body<UNIFORM, LPT>(tile, d.g, a, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.f[TC_PR],
+                              d.f[TC_RU1], d.f[TC_RU2], d.f[TC_RU3], d.f[TC_RHS], d.f[TC_DQ],
+                              d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur, d.luz, d.lu_nt, ntx);
 }
 
 static ImpArgs imp_args(const tc_ctx *c) {
@@ -870,6 +910,21 @@ __global__ __launch_bounds__(256) void k_correct_both(GridDev g, double *__restr
   if (kc == g.nrm) qr[IX(g, g.nr, j, i)] = 0.0;
 }
 
+// Grouped: blockIdx.z = instance * nrm + plane.
+__global__ __launch_bounds__(256) void k_correct_both_g(const StepDesc *__restrict__ tab, int nrm) {
+  const int inst = blockIdx.z / nrm, kc = blockIdx.z - inst * nrm + 1;
+  const StepDesc &d = tab[inst];
+  const GridDev &g = d.g;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nthm || j >= g.nzm) return;
+  double *qr = d.f[TC_QR];
+  correct_velocity_at(g, d.f[TC_QT], qr, d.f[TC_QZ], d.f[TC_DPH], kc, j, i, d.dt, d.al);
+  correct_pressure_at(g, d.f[TC_PR], d.f[TC_DPH], kc, j, i, d.beta);
+  if (kc == 1) qr[IX(g, 1, j, i)] = 0.0;
+  if (kc == g.nrm) qr[IX(g, g.nr, j, i)] = 0.0;
+}
+
 // qr(1,:,:) = qr(nr,:,:) = 0 (CorrectVelocity.F90:46-47)
 __global__ void k_zero_wall_qr(GridDev g, double *__restrict__ qr) {
   const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -941,9 +996,9 @@ struct Col {
   long long lip, ljp;  // in-plane offsets of (j, i+1) and (j+1, i)
   bool ok;
 };
-DEVI Col column(const GridDev &g) {
+DEVI Col column(const GridDev &g, long long blk) {
   Col c;
-  c.l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  c.l = blk * blockDim.x + threadIdx.x;
   c.ok = c.l < g.P;
   const long long l = c.ok ? c.l : 0;
   const int j = (int)(l / g.nthm), i = (int)(l - (long long)j * g.nthm);
@@ -954,12 +1009,11 @@ DEVI Col column(const GridDev &g) {
 
 // CalcMaxCFL.F90:30-43 (WHICH 0) and CheckDivergence.F90:28-40 (WHICH 1)
 template <int WHICH>
-__global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const double *__restrict__ qt,
-                                                         const double *__restrict__ qr,
-                                                         const double *__restrict__ qz,
-                                                         double *__restrict__ part) {
+DEVI void pointmax_body(const int blk, const GridDev &g, const double *__restrict__ qt,
+                        const double *__restrict__ qr, const double *__restrict__ qz,
+                        double *__restrict__ part) {
   double m[1] = {0.0};
-  const Col c = column(g);
+  const Col c = column(g, blk);
   if (c.ok) {
     double r_c = qr[c.l];
     for (int kc = 1; kc <= g.nrm; kc++) {
@@ -978,7 +1032,21 @@ __global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const doubl
     }
   }
   const int ops[1] = {1};
-  block_reduce<1>(m, ops, part + blockIdx.x);
+  block_reduce<1>(m, ops, part + blk);
+}
+
+template <int WHICH>
+__global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const double *__restrict__ qt,
+                                                         const double *__restrict__ qr,
+                                                         const double *__restrict__ qz,
+                                                         double *__restrict__ part) {
+  pointmax_body<WHICH>(blockIdx.x, g, qt, qr, qz, part);
+}
+
+// Grouped CalcMaxCFL partials: blockIdx.y = instance.
+__global__ __launch_bounds__(RED_THREADS) void k_pointmax_cfl_g(const StepDesc *__restrict__ tab) {
+  const StepDesc &d = tab[blockIdx.y];
+  pointmax_body<0>(blockIdx.x, d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.red);
 }
 
 // CalcTorque.F90:36-45 masked wall sums
@@ -1008,7 +1076,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_velstats(GridDev g, const doubl
                                                          const double *__restrict__ qz,
                                                          double pi, double *__restrict__ part) {
   double v[6] = {-100.0, -100.0, -100.0, 0.0, 0.0, 0.0};
-  const Col c = column(g);
+  const Col c = column(g, blockIdx.x);
   if (c.ok) {
     double r_c = qr[c.l];
     for (int kc = 1; kc <= g.nrm; kc++) {
@@ -1033,10 +1101,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_velstats(GridDev g, const doubl
 
 // combine: block q reduces value q over the nb partials (fixed order: strided
 // per-thread sums, then a fixed tree)
-__global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part, int nb, int nv,
-                                                 unsigned opmask, double *__restrict__ out) {
+DEVI void combine_body(const int q, const double *__restrict__ part, int nb, int nv,
+                       unsigned opmask, double *__restrict__ out) {
   __shared__ double sh[256];
-  const int q = blockIdx.x;
   const bool mx = (opmask >> q) & 1u;
   double x = mx ? -1.0e308 : 0.0;
   for (int b = threadIdx.x; b < nb; b += 256) {
@@ -1053,6 +1120,52 @@ __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part
     __syncthreads();
   }
   if (threadIdx.x == 0) out[q] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part, int nb, int nv,
+                                                 unsigned opmask, double *__restrict__ out) {
+  combine_body(blockIdx.x, part, nb, nv, opmask, out);
+}
+
+// Grouped combine: blockIdx.x = value, blockIdx.y = instance; with dyn, only
+// instances logging this step.
+__global__ __launch_bounds__(256) void k_combine_g(const StepDesc *__restrict__ tab,
+                                                   const BatchDyn *__restrict__ dyn, int nb,
+                                                   int nv, unsigned opmask) {
+  if (dyn && !dyn[blockIdx.y].log) return;
+  const StepDesc &d = tab[blockIdx.y];
+  combine_body(blockIdx.x, d.red, nb, nv, opmask, d.red_out);
+}
+
+// CalcMaxCFL floor and the fixed-dt guard of main.F90:156,172-173, one
+// thread per instance: remember the first step with cfl*dt > CFLMAX.
+__global__ void k_cfl_check_g(const StepDesc *__restrict__ tab, const BatchDyn *__restrict__ dyn,
+                              BatchRes *__restrict__ res, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const StepDesc &d = tab[q];
+  const double cfl = fmax(1.0e-8, d.red_out[0]);
+  res[q].cfl = cfl;
+  if (res[q].fail165 == 0 && cfl * d.dt > d.limitcfl) res[q].fail165 = dyn[q].ntime + 1;
+}
+
+// Keep the logging pass's values of the instances that logged this step.
+__global__ void k_diag_store_g(const StepDesc *__restrict__ tab, const BatchDyn *__restrict__ dyn,
+                               BatchRes *__restrict__ res, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n || !dyn[q].log) return;
+  const double *o = tab[q].red_out;
+  for (int v = 0; v < 10; v++) res[q].diag[v] = o[v];
+  res[q].log_ntime = dyn[q].ntime;
+  // main.F90:182,203: 266 when both vmax(1) and vmax(2) exceed VMAX, then
+  // 169 (assigned later, so it wins) when max|div| > RESID
+  int code = 0;
+  if (o[2] > tab[q].limitvel && o[3] > tab[q].limitvel) code = 266;
+  if (fabs(o[1]) > tab[q].resid) code = 169;
+  if (code && res[q].faillog == 0) {
+    res[q].faillog = dyn[q].ntime + 1;
+    res[q].faillog_code = code;
+  }
 }
 
 static unsigned red_blocks(const tc_ctx *c) { return (unsigned)((c->P + RED_THREADS - 1) / RED_THREADS); }
@@ -1097,13 +1210,14 @@ void launch_velocity_stats(tc_ctx *c) {
 // CalcVelocityStats (CalcVelocityStats.F90:35-44) and CalcTorque
 // (CalcTorque.F90:38-43).  v = cfl, |div|, vmax1..3, rms sums 1..3,
 // torque sums inner/outer.
-__global__ __launch_bounds__(RED_THREADS) void k_diagnostics(
-    GridDev g, const double *__restrict__ qt, const double *__restrict__ qr,
-    const double *__restrict__ qz, const unsigned char *__restrict__ icyl,
-    const unsigned char *__restrict__ ocyl, double virotint, double virotext, double del,
-    double deln, double pi, double *__restrict__ part) {
+DEVI void diagnostics_body(const int blk, const GridDev &g, const double *__restrict__ qt,
+                           const double *__restrict__ qr, const double *__restrict__ qz,
+                           const unsigned char *__restrict__ icyl,
+                           const unsigned char *__restrict__ ocyl, double virotint,
+                           double virotext, double del, double deln, double pi,
+                           double *__restrict__ part) {
   double v[10] = {0.0, 0.0, -100.0, -100.0, -100.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const Col c = column(g);
+  const Col c = column(g, blk);
   if (c.ok) {
     double r_c = qr[c.l], t_prev = 0.0;
     for (int kc = 1; kc <= g.nrm; kc++) {
@@ -1138,7 +1252,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_diagnostics(
     }
   }
   const int ops[10] = {1, 1, 1, 1, 1, 0, 0, 0, 0, 0};
-  block_reduce<10>(v, ops, part + 10 * blockIdx.x);
+  block_reduce<10>(v, ops, part + 10 * blk);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_diagnostics(
+    GridDev g, const double *__restrict__ qt, const double *__restrict__ qr,
+    const double *__restrict__ qz, const unsigned char *__restrict__ icyl,
+    const unsigned char *__restrict__ ocyl, double virotint, double virotext, double del,
+    double deln, double pi, double *__restrict__ part) {
+  diagnostics_body(blockIdx.x, g, qt, qr, qz, icyl, ocyl, virotint, virotext, del, deln, pi, part);
+}
+
+// Grouped logging pass: blockIdx.y = instance; instances not logging this
+// step leave at once.
+__global__ __launch_bounds__(RED_THREADS) void k_diagnostics_g(const StepDesc *__restrict__ tab,
+                                                              const BatchDyn *__restrict__ dyn) {
+  if (!dyn[blockIdx.y].log) return;
+  const StepDesc &d = tab[blockIdx.y];
+  diagnostics_body(blockIdx.x, d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.icyl, d.ocyl,
+                   d.virotint, d.virotext, d.del, d.deln, d.pi, d.red);
 }
 
 void launch_diagnostics(tc_ctx *c) {
@@ -1171,6 +1303,56 @@ void launch_copy(tc_ctx *c, int dst, int src) {
   hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (const double2 *)c->f[src],
                      (double2 *)c->f[dst], n2, c->f[src], c->f[dst], c->N);
   timer_end(c);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches over the instances of one shape group
+// ---------------------------------------------------------------------------
+
+template <bool U, int LPT>
+static void launch_fwd_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
+                         hipStream_t st) {
+  const int ntx = (c->nthm + FT_I - 1) / FT_I, ntz = (c->nzm + 4 * LPT - 1) / (4 * LPT);
+  hipLaunchKernelGGL((k_helm_fwd_g<U, LPT>), dim3((unsigned)(ntx * ntz), (unsigned)n), dim3(256), 0,
+                     st, tab, dyn, ns, ntx, ntz);
+}
+
+void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
+                              hipStream_t st) {
+  if (c->masks_uniform_one) {
+    if (c->helm_lpt == 2) launch_fwd_g<true, 2>(c, tab, dyn, ns, n, st);
+    else launch_fwd_g<true, 1>(c, tab, dyn, ns, n, st);
+  } else {
+    if (c->helm_lpt == 2) launch_fwd_g<false, 2>(c, tab, dyn, ns, n, st);
+    else launch_fwd_g<false, 1>(c, tab, dyn, ns, n, st);
+  }
+  hipLaunchKernelGGL(k_helm_bwd_g, dim3(line_blocks(c), (unsigned)n), dim3(256), 0, st, tab);
+}
+
+void launch_correct_both_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t st) {
+  const dim3 grid((unsigned)((c->nthm + 63) / 64), (unsigned)((c->nzm + 3) / 4),
+                  (unsigned)(c->nrm * n));
+  hipLaunchKernelGGL(k_correct_both_g, grid, dim3(64, 4), 0, st, tab, c->nrm);
+}
+
+void launch_cfl_check_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, BatchRes *res, int n,
+                        hipStream_t st) {
+  const unsigned nb = red_blocks(c);
+  hipLaunchKernelGGL(k_pointmax_cfl_g, dim3(nb, (unsigned)n), dim3(RED_THREADS), 0, st, tab);
+  hipLaunchKernelGGL(k_combine_g, dim3(1, (unsigned)n), dim3(256), 0, st, tab,
+                     (const BatchDyn *)nullptr, (int)nb, 1, 1u);
+  hipLaunchKernelGGL(k_cfl_check_g, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, tab, dyn, res,
+                     n);
+}
+
+void launch_diagnostics_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, BatchRes *res, int n,
+                          hipStream_t st) {
+  const unsigned nb = red_blocks(c);
+  hipLaunchKernelGGL(k_diagnostics_g, dim3(nb, (unsigned)n), dim3(RED_THREADS), 0, st, tab, dyn);
+  hipLaunchKernelGGL(k_combine_g, dim3(10, (unsigned)n), dim3(256), 0, st, tab, dyn, (int)nb, 10,
+                     0x1Fu);
+  hipLaunchKernelGGL(k_diag_store_g, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, tab, dyn,
+                     res, n);
 }
 
 }  // namespace tc

@@ -201,13 +201,13 @@ struct DivIn {
 };
 
 template <int ME, bool DIV, bool STAGE = false>
-DEVI void theta_fwd_body(const double *__restrict__ in, const DivIn &dv, const GridDev &g,
-                         double2 *__restrict__ spec, const FFTPlan &pl,
+DEVI void theta_fwd_body(const long long pb, const double *__restrict__ in, const DivIn &dv,
+                         const GridDev &g, double2 *__restrict__ spec, const FFTPlan &pl,
                          const double2 *__restrict__ tw, long long nrows, int nth2, int sstr) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = 256 / L;
   const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
-  const long long pair = (long long)blockIdx.x * L + ln;
+  const long long pair = pb * L + ln;
   const long long r0 = 2 * pair, r1 = r0 + 1;
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
   double2 *buf = lds;
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_fft_theta_fwd(const double *__restrict_
                                                        long long nrows, int nth2, int sstr) {
   const DivIn dv{};
   const GridDev g{};
-  theta_fwd_body<ME, false>(in, dv, g, spec, pl, tw, nrows, nth2, sstr);
+  theta_fwd_body<ME, false>(blockIdx.x, in, dv, g, spec, pl, tw, nrows, nth2, sstr);
 }
 
 template <int ME, bool STAGE>
@@ -280,7 +280,16 @@ __global__ __launch_bounds__(256) void k_div_fft_theta_fwd(DivIn dv, GridDev g,
                                                            double2 *__restrict__ spec, FFTPlan pl,
                                                            const double2 *__restrict__ tw,
                                                            long long nrows, int nth2, int sstr) {
-  theta_fwd_body<ME, true, STAGE>(nullptr, dv, g, spec, pl, tw, nrows, nth2, sstr);
+  theta_fwd_body<ME, true, STAGE>(blockIdx.x, nullptr, dv, g, spec, pl, tw, nrows, nth2, sstr);
+}
+
+// Grouped: blockIdx.y = instance.
+template <int ME>
+__global__ __launch_bounds__(256) void k_div_fft_theta_fwd_g(const StepDesc *__restrict__ tab) {
+  const StepDesc &d = tab[blockIdx.y];
+  const DivIn dv{d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.usdt};
+  theta_fwd_body<ME, true, false>(blockIdx.x, nullptr, dv, d.g, d.spec, d.pt, d.twt,
+                                  (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
 }
 
 // c2r along theta for row pairs; the imaginary parts of ktheta = 0 and
@@ -289,14 +298,13 @@ __global__ __launch_bounds__(256) void k_div_fft_theta_fwd(DivIn dv, GridDev g,
 // Z[n-k] = conj(A[k]) + i conj(B[k]) go to LDS together; the last pass
 // writes the two real rows straight to memory.
 template <int ME>
-__global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict__ spec,
-                                                       double *__restrict__ out, FFTPlan pl,
-                                                       const double2 *__restrict__ tw,
-                                                       long long nrows, int nth2, int sstr) {
+DEVI void theta_inv_body(const long long pb, const double2 *__restrict__ spec,
+                         double *__restrict__ out, const FFTPlan &pl,
+                         const double2 *__restrict__ tw, long long nrows, int nth2, int sstr) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = 256 / L;
   const int ln = threadIdx.x / T, t = threadIdx.x - ln * T;
-  const long long pair = (long long)blockIdx.x * L + ln;
+  const long long pair = pb * L + ln;
   const long long r0 = 2 * pair, r1 = r0 + 1;
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
   double2 *buf = lds;
@@ -320,6 +328,22 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
   fft_line<ME, false, true>(buf, off, 1, pl, t, T, tw, 1.0, ld, st);
 }
 
+template <int ME>
+__global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict__ spec,
+                                                       double *__restrict__ out, FFTPlan pl,
+                                                       const double2 *__restrict__ tw,
+                                                       long long nrows, int nth2, int sstr) {
+  theta_inv_body<ME>(blockIdx.x, spec, out, pl, tw, nrows, nth2, sstr);
+}
+
+// Grouped: blockIdx.y = instance; output into dph.
+template <int ME>
+__global__ __launch_bounds__(256) void k_fft_theta_inv_g(const StepDesc *__restrict__ tab) {
+  const StepDesc &d = tab[blockIdx.y];
+  theta_inv_body<ME>(blockIdx.x, d.spec, d.f[TC_DPH], d.pt, d.twt, (long long)d.g.nrm * d.g.nzm,
+                     d.g.nth2, d.g.sstr);
+}
+
 // Workgroups are dispatched round-robin over the XCDs; hand each XCD a
 // contiguous run of tiles so tiles sharing cache lines share one L2.
 DEVI int xcd_tile(int b, int nb) {
@@ -334,13 +358,11 @@ DEVI int xcd_tile(int b, int nb) {
 // so global and LDS accesses are both contiguous across lanes.  Spectral rows
 // are sstr complex apart (a multiple of 8, i.e. of 128 bytes).
 template <int ME, int NT>
-__global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPlan pl,
-                                              const double2 *__restrict__ tw, int nth2, int sstr,
-                                              int ntile, double sg) {
+DEVI void fft_z_body(const int id, double2 *__restrict__ spec, const FFTPlan &pl,
+                     const double2 *__restrict__ tw, int nth2, int sstr, int ntile, double sg) {
   extern __shared__ double2 lds[];
   const int n = pl.n, L = pl.lines, T = NT / L;
   const int c = threadIdx.x % L, t = threadIdx.x / L;
-  const int id = xcd_tile(blockIdx.x, gridDim.x);
   const int c0 = (id % ntile) * L;
   double2 *col = spec + (long long)(id / ntile) * n * sstr + c0 + c;
   const bool ok = c0 + c < nth2;
@@ -349,6 +371,23 @@ __global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPla
     if (ok) col[(long long)z * sstr] = v;
   };
   fft_line<ME, true, true>(lds, c, L, pl, t, T, tw, sg, ld, st);
+}
+
+template <int ME, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPlan pl,
+                                              const double2 *__restrict__ tw, int nth2, int sstr,
+                                              int ntile, double sg) {
+  fft_z_body<ME, NT>(xcd_tile(blockIdx.x, gridDim.x), spec, pl, tw, nth2, sstr, ntile, sg);
+}
+
+// Grouped: the XCD remap runs over the flattened (instance, tile) index.
+template <int ME, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z_g(const StepDesc *__restrict__ tab, int ntiles,
+                                                int ntile, double sg) {
+  const int lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int inst = lin / ntiles, id = lin - inst * ntiles;
+  const StepDesc &d = tab[inst];
+  fft_z_body<ME, NT>(id, d.spec, d.pz, d.twz, d.g.nth2, d.g.sstr, ntile, sg);
 }
 
 // SolvePressureCorrection.F90:127-148: radial tridiagonal per (kz, ktheta)
@@ -369,18 +408,18 @@ __global__ __launch_bounds__(NT) void k_fft_z(double2 *__restrict__ spec, FFTPla
 //   blockDim = (LB lines, PT partitions)
 //   LDS = LB*2PT*4 doubles (reduced systems) + 3*nrm doubles (gauge line)
 template <int PQ>  // max rows per thread
-__global__ __launch_bounds__(512) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
-                                                      double inv_norm) {
+DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restrict__ spec,
+                       double inv_norm) {
   extern __shared__ double red[];  // [LB][2*PT][4]: A, C, D.x, D.y
   const int LB = blockDim.x, PT = blockDim.y;
   const int lx = threadIdx.x, pp = threadIdx.y;
   const int tid = pp * LB + lx, nthr = LB * PT;
   const long long M = (long long)g.nzm * g.sstr;
-  const long long m = (long long)blockIdx.x * LB + lx;
+  const long long m = blk * LB + lx;
   const int n = g.nrm;
   double2 *gline = (double2 *)(red + (size_t)LB * 2 * PT * 4);  // gauge line, n values
   double *gcp = (double *)(gline + n);                          // its multipliers
-  const bool gblock = (blockIdx.x == 0);
+  const bool gblock = (blk == 0);
   if (gblock)
     for (int k = tid; k < n; k += nthr) {
       const double2 v = spec[(long long)k * M];
@@ -538,6 +577,19 @@ __global__ __launch_bounds__(512) void k_poisson_tridiag(GridDev g, double2 *__r
     for (int k = tid; k < n; k += nthr) spec[(long long)k * M] = gline[k];
 }
 
+template <int PQ>
+__global__ __launch_bounds__(512) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
+                                                         double inv_norm) {
+  poisson_body<PQ>(blockIdx.x, g, spec, inv_norm);
+}
+
+// Grouped: blockIdx.y = instance.
+template <int PQ>
+__global__ __launch_bounds__(512) void k_poisson_tridiag_g(const StepDesc *__restrict__ tab) {
+  const StepDesc &d = tab[blockIdx.y];
+  poisson_body<PQ>(blockIdx.x, d.g, d.spec, d.inv_norm);
+}
+
 // Launch shape of k_poisson_tridiag for a line of n rows: PT partitions of at
 // most PQ rows (and at least 2), LB lines per block with LB*PT <= 512
 // (<= 256 for PQ = 13, so that three blocks share a CU).
@@ -690,6 +742,65 @@ void launch_poisson(tc_ctx *c, bool with_divergence, double usdt) {
   timer_begin(c, "fft_theta_inv");
   theta_pass(c, false);
   timer_end(c);
+}
+
+// Grouped Poisson solve of n same-shape instances: divergence fused into the
+// forward theta FFT, z FFT, radial solve, inverse z and theta FFTs.
+void launch_poisson_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t st) {
+  const FFTPlan &pt = c->plan_t, &pz = c->plan_z;
+  const long long nrows = (long long)c->nrm * c->nzm;
+  const unsigned tb = (unsigned)(((nrows + 1) / 2 + pt.lines - 1) / pt.lines);
+  const size_t lt = fft_lds_bytes(pt), lz = fft_lds_bytes(pz);
+  const dim3 gt(tb, (unsigned)n);
+#define TC_TG(K, ME) hipLaunchKernelGGL(K<ME>, gt, dim3(256), lt, st, tab)
+  switch (pt.me) {
+    case 4: TC_TG(k_div_fft_theta_fwd_g, 4); break;
+    case 8: TC_TG(k_div_fft_theta_fwd_g, 8); break;
+    case 16: TC_TG(k_div_fft_theta_fwd_g, 16); break;
+    default: TC_TG(k_div_fft_theta_fwd_g, 32); break;
+  }
+  const int ntile = (c->nth2 + pz.lines - 1) / pz.lines;
+  const int ntiles = ntile * c->nrm;
+  for (int pass = 0; pass < 3; pass++) {
+    if (pass == 1) {
+      int ptn, lb;
+      poisson_part_shape(c->nrm, c->poisson_pq, ptn, lb);
+      const size_t lds = sizeof(double) * (4 * 2 * (size_t)ptn * lb + 3 * (size_t)c->nrm);
+      const long long M = (long long)c->nzm * c->sstr;
+      const dim3 grid((unsigned)((M + lb - 1) / lb), (unsigned)n), block(lb, ptn);
+      if (c->poisson_pq == 13)
+        hipLaunchKernelGGL(k_poisson_tridiag_g<13>, grid, block, lds, st, tab);
+      else
+        hipLaunchKernelGGL(k_poisson_tridiag_g<26>, grid, block, lds, st, tab);
+      continue;
+    }
+    const double sg = pass == 0 ? -1.0 : 1.0;
+    const dim3 gz((unsigned)ntiles, (unsigned)n);
+#define TC_ZG(ME, NT) hipLaunchKernelGGL((k_fft_z_g<ME, NT>), gz, dim3(NT), lz, st, tab, ntiles, ntile, sg)
+    if (pz.nt == 512) {
+      switch (pz.me) {
+        case 4: TC_ZG(4, 512); break;
+        case 8: TC_ZG(8, 512); break;
+        case 16: TC_ZG(16, 512); break;
+        default: TC_ZG(32, 512); break;
+      }
+    } else {
+      switch (pz.me) {
+        case 4: TC_ZG(4, 256); break;
+        case 8: TC_ZG(8, 256); break;
+        case 16: TC_ZG(16, 256); break;
+        default: TC_ZG(32, 256); break;
+      }
+    }
+#undef TC_ZG
+  }
+  switch (pt.me) {
+    case 4: TC_TG(k_fft_theta_inv_g, 4); break;
+    case 8: TC_TG(k_fft_theta_inv_g, 8); break;
+    case 16: TC_TG(k_fft_theta_inv_g, 16); break;
+    default: TC_TG(k_fft_theta_inv_g, 32); break;
+  }
+#undef TC_TG
 }
 
 }  // namespace tc

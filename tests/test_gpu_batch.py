@@ -1,0 +1,176 @@
+"""Batched sweep: grouped launches over many instances.
+
+- every instance of a batch follows exactly the single-instance fused path
+  (tc_step): fields bit-identical after several steps, with the inner
+  cylinder oscillating;
+- hipGraph replay gives the same fields as eager launches, replay after
+  replay, with host synchronisation between replays;
+- an instance gives the same result alone and in a mix of other shapes;
+- the exit codes and their steps are those of the main.F90 loop;
+- the autotuner returns a schedule that runs and matches the default one.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("qt", "qr", "qz", "pr")
+
+
+def _params(tc, nthm, nzm, nrm, **kw):
+    p = tc.configs.scaled(tc.configs.small_64(), nthm, nzm, nrm)
+    p.update(idtv=0, dtmax=2e-3, dtmin=2e-3, limitcfl=1e6, tout=1e30, tmax=1e30,
+             sizepert=0.1, omegapert=0.5, ntst=1000)
+    p.update(kw)
+    return p
+
+
+def _start(tc, p, seed):
+    """Projected perturbed Couette start, computed once on a single context."""
+    s = tc.Solver(p)
+    qt, qr, qz = tc.init_perturbed_couette(p, seed, 1e-2)
+    s.set("qt", qt); s.set("qr", qr); s.set("qz", qz)
+    s.project()
+    st = {f: s.get(f) for f in ("qt", "qr", "qz")}
+    s.close()
+    return st
+
+
+def _load(solver, st):
+    for f, a in st.items():
+        solver.set(f, a)
+
+
+def _mix(tc):
+    return [
+        (_params(tc, 32, 24, 33, ren=800.0), 11),
+        (_params(tc, 32, 24, 33, ren=1500.0, rext=1.6, virotext=-0.2), 12),
+        (_params(tc, 48, 32, 25, ren=1000.0, lamb=2.0), 13),
+        (_params(tc, 24, 40, 17, ren=600.0, rext=1.25), 14),
+    ]
+
+
+def _single(tc, p, st, nsteps):
+    s = tc.Solver(p)
+    _load(s, st)
+    diags = []
+    for n in range(nsteps):
+        code, d = s.step(n)
+        diags.append((code, d))
+    out = {f: s.get(f) for f in FIELDS}
+    s.close()
+    return out, diags
+
+
+def test_batch_matches_single_instance_bitwise(tc):
+    nsteps = 5
+    mix = _mix(tc)
+    starts = [_start(tc, p, seed) for p, seed in mix]
+    with tc.Batch() as b:
+        ids = []
+        for (p, _), st in zip(mix, starts):
+            i = b.add(p)
+            _load(b.solver(i), st)
+            ids.append(i)
+        ng, launches = b.groups()
+        assert ng == 3, ng
+        assert b.advance(nsteps, graph=True, sync_every=2) == len(mix)
+        for (p, _), st, i in zip(mix, starts, ids):
+            ref, diags = _single(tc, p, st, nsteps)
+            got = b.solver(i)
+            for f in FIELDS:
+                a = got.get(f)
+                assert np.array_equal(a, ref[f]), (i, f, np.abs(a - ref[f]).max())
+            steps, code, d = b.status(i)
+            assert steps == nsteps and code == 0
+            # the last logging step is ntime = 1
+            d1 = diags[1][1]
+            for k in ("dmax", "torque_in", "torque_out", "vmax1", "vmax2", "vmax3"):
+                assert d[k] == d1[k], (k, d[k], d1[k])
+
+
+def test_batch_graph_replays_match_eager(tc):
+    mix = _mix(tc)[:3]
+    starts = [_start(tc, p, seed) for p, seed in mix]
+    results = []
+    for graph in (False, True):
+        with tc.Batch() as b:
+            ids = []
+            for (p, _), st in zip(mix, starts):
+                i = b.add(p)
+                _load(b.solver(i), st)
+                ids.append(i)
+            snaps = []
+            for rep in range(3):  # three advances: the graph is replayed, host syncs between
+                b.advance(2, graph=graph, sync_every=1)
+                snaps.append([b.solver(i).get("qt") for i in ids])
+            results.append(snaps)
+    for rep in range(3):
+        for q in range(len(mix)):
+            assert np.array_equal(results[0][rep][q], results[1][rep][q]), (rep, q)
+
+
+def test_same_alone_and_in_the_mix(tc):
+    mix = _mix(tc)
+    p0, seed0 = mix[1]
+    st0 = _start(tc, p0, seed0)
+    out = []
+    for others in (False, True):
+        with tc.Batch() as b:
+            i0 = b.add(p0)
+            _load(b.solver(i0), st0)
+            if others:
+                for p, seed in mix[:1] + mix[2:]:
+                    i = b.add(p)
+                    _load(b.solver(i), _start(tc, p, seed))
+            b.advance(4, graph=True, sync_every=4)
+            out.append({f: b.solver(i0).get(f) for f in FIELDS})
+    for f in FIELDS:
+        assert np.array_equal(out[0][f], out[1][f]), f
+
+
+def test_batch_exit_codes_match_main_loop(tc):
+    base = _params(tc, 32, 24, 33)
+    cases = [dict(ntst=3), dict(tmax=0.0045), dict(limitcfl=1e-6), dict(ntst=6, resid=1e-30)]
+    st = _start(tc, base, 21)
+    with tc.Batch() as b:
+        ids = []
+        for kw in cases:
+            p = dict(base, **kw)
+            i = b.add(p)
+            _load(b.solver(i), st)
+            ids.append((i, p))
+        b.advance(10, graph=True, sync_every=3)
+        for i, p in ids:
+            steps, code, _ = b.status(i)
+            s = tc.Solver(p)
+            _load(s, st)
+            want = None
+            for n in range(10):
+                c, _ = s.step(n)
+                if c:
+                    want = (n + 1, c)
+                    break
+            s.close()
+            assert (steps, code) == want, (p, steps, code, want)
+
+
+def test_autotune_schedule_runs_and_matches_default(tc):
+    p = _params(tc, 64, 48, 33)
+    t = tc.autotune(p, steps=2)
+    assert t["helm_lpt"] in (1, 2) and t["poisson_pq"] in (13, 26)
+    assert t["fft_z_nt"] in (256, 512) and t["fft_pad"] in (1, -1)
+    st = _start(tc, p, 31)
+    a = tc.Solver(p, tuning=t)
+    b = tc.Solver(p)
+    _load(a, st)
+    _load(b, st)
+    for n in range(3):
+        a.step(n)
+        b.step(n)
+    for f in ("qt", "qr", "qz"):
+        x, y = a.get(f), b.get(f)
+        assert np.linalg.norm(x - y) <= 1e-12 * np.linalg.norm(y), f
+    assert a.tuning()["poisson_pq"] == t["poisson_pq"]
+    a.close()
+    b.close()

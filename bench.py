@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -154,6 +156,133 @@ def cpu_leg(p, seed, amp, max_seconds):
     return n, t_steps, threads
 
 
+def _alone_step_ms(tc, p, fields, steps=8, warmup=2, tuning=None):
+    """Per-step time of one job alone on the plain single-instance path
+    (tc_step: CFL, dt rule, fused RK3 step, logging diagnostics)."""
+    from importlib import import_module
+    sw = import_module("taylor-couette-afid_amd.sweep")
+    t0 = time.perf_counter()
+    s = tc.Solver(p, tuning=tuning)
+    sw.load_and_project(s, fields)
+    s.synchronize()
+    setup = time.perf_counter() - t0
+    for n in range(warmup):
+        s.step(n)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for n in range(warmup, warmup + steps):
+        s.step(n)
+    s.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    s.close()
+    return ms, setup
+
+
+def sweep_leg(tc, steps_timed):
+    """Batched sweep of 48 uneven instances: per-step time of the grouped,
+    graph-replayed step of all instances, against the same jobs run one at a
+    time on the plain path (per-shape step times, whole-job time summed)."""
+    from importlib import import_module
+    sw = import_module("taylor-couette-afid_amd.sweep")
+    jobs = sw.sweep_jobs()
+    prepared = []
+    for p, seed in jobs:
+        p = dict(p)
+        prepared.append((p, sw.start_fields(tc, p, seed)))
+    shapes = sorted({(p["nthm"], p["nzm"], p["nrm"]) for p, _ in prepared})
+    tuning = {}
+    for sh in shapes:
+        p = next(q for q, _ in prepared if (q["nthm"], q["nzm"], q["nrm"]) == sh)
+        tuning[sh] = tc.autotune(p, steps=3)
+    print(f"[bench] sweep: {len(jobs)} instances, shapes {shapes}, tuning {tuning}",
+          file=sys.stderr, flush=True)
+    # one job at a time on the plain path, per shape
+    alone = {}
+    for sh in shapes:
+        p, f = next((q, f) for q, f in prepared if (q["nthm"], q["nzm"], q["nrm"]) == sh)
+        alone[sh] = _alone_step_ms(tc, p, f)
+    b = tc.Batch()
+    for p, f in prepared:
+        iid = b.add(p, tuning[(p["nthm"], p["nzm"], p["nrm"])])
+        sw.load_and_project(b.solver(iid), f)
+    del prepared
+    ngroups, launches = b.groups()
+    b.advance(2, graph=True, sync_every=2)
+    t0 = time.perf_counter()
+    b.advance(steps_timed, graph=True, sync_every=steps_timed)
+    el = time.perf_counter() - t0
+    codes = [b.status(i)[1] for i in range(len(jobs))]
+    b.close()
+    if any(codes):
+        raise RuntimeError(f"sweep instances ended early: {codes}")
+    cells = sum(p["nthm"] * p["nzm"] * p["nrm"] for p, _ in jobs)
+    steps_job = jobs[0][0]["ntst"] + 1
+    ms = 1e3 * el / steps_timed
+    jobs_h = len(jobs) / (steps_job * ms / 1e3) * 3600
+    alone_s = sum(alone[(p["nthm"], p["nzm"], p["nrm"])][0] * steps_job / 1e3 +
+                  alone[(p["nthm"], p["nzm"], p["nrm"])][1] for p, _ in jobs)
+    alone_jobs_h = len(jobs) / alone_s * 3600
+    return {
+        "workload": "48 uneven FP64 instances (64^2x65, 128^2x97, 192x128x129, 256^2x193), "
+                    "eta, 1/NU, VIROTEXT/VIROTINT, LAMB seeded; fixed dt at CFL 0.8; 300-step jobs",
+        "instances": len(jobs), "groups": ngroups, "launches_per_step": launches,
+        "steps_timed": steps_timed, "ms_per_step": round(ms, 3),
+        "mpt_steps_per_s": round(cells * steps_timed / el / 1e6, 1),
+        "jobs_per_h": round(jobs_h, 1),
+        "one_at_a_time_jobs_per_h": round(alone_jobs_h, 1),
+        "speedup_vs_one_at_a_time": round(jobs_h / alone_jobs_h, 2),
+        "one_at_a_time_ms_per_step": {f"{a}x{b}x{c}": round(v[0], 3) for (a, b, c), v in alone.items()},
+    }
+
+
+def served_leg(tc, njobs):
+    """Served sweep: Poisson arrivals (2 jobs/s) of 1000-step jobs through the
+    continuous-batching server, against a FIFO of the same jobs run one at a
+    time on the plain path (service time = measured setup + 1000 x measured
+    step time of the job's shape)."""
+    from importlib import import_module
+    sw = import_module("taylor-couette-afid_amd.sweep")
+    jobs = sw.served_jobs(njobs)
+    shapes = sorted({(p["nthm"], p["nzm"], p["nrm"]) for _, p, _ in jobs})
+    tuning, alone = {}, {}
+    for sh in shapes:
+        _, p, seed = next(j for j in jobs if (j[1]["nthm"], j[1]["nzm"], j[1]["nrm"]) == sh)
+        p = dict(p)
+        f = sw.start_fields(tc, p, seed)
+        tuning[sh] = tc.autotune(p, steps=3)
+        alone[sh] = _alone_step_ms(tc, p, f)
+        del f
+    srv = sw.Server(tc, tuning=tuning)
+    rec = srv.run(jobs)
+    codes = [r["code"] for r in rec]
+    if any(c != 555 for c in codes):
+        raise RuntimeError(f"served jobs ended with {codes}")
+    ttr = np.array([r["done"] - r["arrival"] for r in rec])
+    makespan = max(r["done"] for r in rec)
+    cell_steps = sum(r["cells"] * r["steps"] for r in rec)
+    # the same arrivals, one job at a time
+    end, ttr1 = 0.0, []
+    for t, p, _ in sorted(jobs, key=lambda j: j[0]):
+        sh = (p["nthm"], p["nzm"], p["nrm"])
+        ms, setup = alone[sh]
+        start = max(t, end)
+        end = start + setup + (p["ntst"] + 1) * ms / 1e3
+        ttr1.append(end - t)
+    ttr1 = np.array(ttr1)
+    return {
+        "workload": f"{njobs} jobs, Poisson arrivals 2/s, grids 64^2x65..384^2x289, "
+                    "eta/Re/rotation ratio/oscillation seeded, 1000 RK3 steps each",
+        "jobs": njobs, "p50_s": round(float(np.percentile(ttr, 50)), 2),
+        "p99_s": round(float(np.percentile(ttr, 99)), 2),
+        "jobs_per_h": round(njobs / makespan * 3600, 1),
+        "mpt_steps_per_s": round(cell_steps / makespan / 1e6, 1),
+        "one_at_a_time": {"p50_s": round(float(np.percentile(ttr1, 50)), 2),
+                          "p99_s": round(float(np.percentile(ttr1, 99)), 2),
+                          "jobs_per_h": round(njobs / end * 3600, 1)},
+        "speedup_jobs_per_h": round((njobs / makespan) / (njobs / end), 2),
+    }
+
+
 def _kernel_short(name):
     """'void tc::k_helm_fwd<true, 2>(tc::GridDev, ...)' -> 'k_helm_fwd<true, 2>'."""
     name = name.replace("void ", "").split("(")[0]
@@ -260,6 +389,10 @@ def main():
     ap.add_argument("--workload", default="production_512")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=30,
+                    help="timed steps of the batched 48-instance sweep (0: skip)")
+    ap.add_argument("--served-jobs", type=int, default=12,
+                    help="jobs of the served sweep (0: skip)")
     ap.add_argument("--summarize-profile", metavar="OUT",
                     help="write the profiles/ summaries from rocprofv3 CSV output in OUT and exit")
     a = ap.parse_args()
@@ -321,6 +454,12 @@ def main():
                      "bytes_per_launch": kb[dom],
                      "copy_kernel_gbs": round(copy_gbs, 1)},
     }
+    if a.sweep_steps > 0:
+        out["sweep"] = sweep_leg(tc, a.sweep_steps)
+        print(f"[bench] sweep {out['sweep']}", file=sys.stderr, flush=True)
+    if a.served_jobs > 0:
+        out["served"] = served_leg(tc, a.served_jobs)
+        print(f"[bench] served {out['served']}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         n, t, threads = cpu_leg(p, seed, amp, a.cpu_seconds)
         out["cpu_baseline"] = {"value": round(npts * n / t / 1e6, 4), "unit": "Mpt\u00b7RK3-steps/s",

@@ -435,7 +435,9 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
     std::vector<double> h(stride * nv, 0.0);
     for (int q = 0; q < nv; q++) std::copy(vs[q]->begin(), vs[q]->end(), h.begin() + q * stride);
     TRY(dev_alloc(c, (void **)&c->gvbuf, sizeof(double) * h.size()));
-    HTRY(hipMemcpy(c->gvbuf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    HTRY(hipMemcpyAsync(c->gvbuf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                        c->stream));
+    HTRY(hipStreamSynchronize(c->stream));
     const double **dst[] = {&c->gd.rc,   &c->gd.rm,    &c->gd.usrc,  &c->gd.usrm,  &c->gd.udx2rm,
                             &c->gd.udx2m, &c->gd.udx2c, &c->gd.rudx2c, &c->gd.udx1vis1,
                             &c->gd.udx1vis2, &c->gd.udx1qm, &c->gd.pvol, &c->gd.ap1j,
@@ -483,10 +485,14 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
     std::vector<double> tw;
     tc::fill_twiddles(c->nthm, tw);
     TRY(dev_alloc(c, (void **)&c->tw_t, sizeof(double) * tw.size()));
-    HTRY(hipMemcpy(c->tw_t, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    HTRY(hipMemcpyAsync(c->tw_t, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice,
+                        c->stream));
+    HTRY(hipStreamSynchronize(c->stream));
     tc::fill_twiddles(c->nzm, tw);
     TRY(dev_alloc(c, (void **)&c->tw_z, sizeof(double) * tw.size()));
-    HTRY(hipMemcpy(c->tw_z, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    HTRY(hipMemcpyAsync(c->tw_z, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice,
+                        c->stream));
+    HTRY(hipStreamSynchronize(c->stream));
   }
   // reductions
   // per-block partials of the reductions: one block of 256 per (j, i) column group
@@ -635,7 +641,9 @@ int tc_upload_field(tc_ctx *c, int field, const double *host, int layout) {
   c->cfl_valid = false;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (layout == TC_LAYOUT_DEVICE) {
-    HIPCHK(hipMemcpy(c->f[field], host, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(c->f[field], host, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice,
+                           c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
   }
   if (layout != TC_LAYOUT_FORTRAN) return fail("unknown layout");
@@ -646,7 +654,9 @@ int tc_upload_field(tc_ctx *c, int field, const double *host, int layout) {
       const double *src = host + ((size_t)j + (size_t)nzm * i) * nr;
       for (int k = 0; k < nr; k++) d[(size_t)k * c->P + (size_t)j * nthm + i] = src[k];
     }
-  HIPCHK(hipMemcpy(c->f[field], d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(c->f[field], d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice,
+                         c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -656,12 +666,16 @@ int tc_download_field(tc_ctx *c, int field, double *host, int layout) {
   if (!host) return fail("null host pointer");
   HIPCHK(hipStreamSynchronize(c->stream));
   if (layout == TC_LAYOUT_DEVICE) {
-    HIPCHK(hipMemcpy(host, c->f[field], sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(host, c->f[field], sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
   }
   if (layout != TC_LAYOUT_FORTRAN) return fail("unknown layout");
   std::vector<double> d((size_t)c->N);
-  HIPCHK(hipMemcpy(d.data(), c->f[field], sizeof(double) * d.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(d.data(), c->f[field], sizeof(double) * d.size(), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   const int nr = c->nr, nzm = c->nzm, nthm = c->nthm;
   for (int i = 0; i < nthm; i++)
     for (int j = 0; j < nzm; j++) {

@@ -116,9 +116,14 @@ int harvest(tc_batch *b) {
   for (Group &g : b->groups) {
     const int n = (int)g.ids.size();
     g.res_host.resize((size_t)n);
-    BCHK(hipMemcpy(g.res_host.data(), g.res, sizeof(BatchRes) * n, hipMemcpyDeviceToHost));
+    BCHK(hipMemcpyAsync(g.res_host.data(), g.res, sizeof(BatchRes) * n, hipMemcpyDeviceToHost,
+                        b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
     for (int q = 0; q < n; q++) {
       Inst &I = b->inst[(size_t)g.ids[(size_t)q]];
+      // removed since the groups were built (they are rebuilt before the
+      // next step)
+      if (!I.live || !I.c) continue;
       const BatchRes &r = g.res_host[(size_t)q];
       tc_ctx *c = I.c;
       if (r.log_ntime >= 0) {
@@ -227,7 +232,9 @@ int rebuild(tc_batch *b) {
         h[(size_t)q] = make_desc(c, ns);
       }
       BCHK(hipMalloc((void **)&g.tab[ns - 1], sizeof(StepDesc) * n));
-      BCHK(hipMemcpy(g.tab[ns - 1], h.data(), sizeof(StepDesc) * n, hipMemcpyHostToDevice));
+      BCHK(hipMemcpyAsync(g.tab[ns - 1], h.data(), sizeof(StepDesc) * n, hipMemcpyHostToDevice,
+                          b->stream));
+      BCHK(hipStreamSynchronize(b->stream));
     }
     BCHK(hipMalloc((void **)&g.dyn, sizeof(BatchDyn) * n));
     BCHK(hipMalloc((void **)&g.res, sizeof(BatchRes) * n));
@@ -236,7 +243,8 @@ int rebuild(tc_batch *b) {
       std::memset(&r, 0, sizeof r);
       r.log_ntime = -1;
     }
-    BCHK(hipMemcpy(g.res, r0.data(), sizeof(BatchRes) * n, hipMemcpyHostToDevice));
+    BCHK(hipMemcpyAsync(g.res, r0.data(), sizeof(BatchRes) * n, hipMemcpyHostToDevice, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
     g.dyn_off = total;
     total += (size_t)n;
     for (int q = 0; q < n; q++) {

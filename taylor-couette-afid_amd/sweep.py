@@ -1,0 +1,329 @@
+"""Parameter sweeps of many Taylor-Couette instances on one device.
+
+Two workloads:
+
+- ``sweep_jobs``: the batched sweep, 48 independent instances of uneven size
+  (grids from {64^2x65, 128^2x97, 192x128x129, 256^2x193}), eta uniform in
+  [0.5, 0.909], 1/NU log-uniform in [1e3, 1e4], VIROTEXT/VIROTINT uniform in
+  [-0.5, 0.2], LAMB in {2, 4, 8}, each from a seeded 1e-2 perturbation of its
+  own Couette profile, fixed dt from a CFL = 0.8 estimate (capped where the
+  reference's pressure update would be unstable, see ``pressure_stable_dt``),
+  300 RK3 steps.
+- ``served_jobs``: the served sweep, Poisson arrivals (mean 2 jobs/s) of the
+  same distribution over grids 64^2x65 .. 384^2x289, plus an inner-cylinder
+  oscillation of amplitude in [0, 0.2] and frequency in [0.01, 0.1], 1000 RK3
+  steps each.
+
+Values the workload descriptions leave open: RINT = 1 (so REXT = 1/eta),
+ALX3D = 2 pi, VIROTINT = 1, STR3 = 1.5 with ISTR = 4, RESID = 1e-6.
+
+``Server`` is the request loop of the served form.  Arrived jobs wait in
+arrival order; a job is admitted when a slot is free, its device footprint
+fits the HBM budget and the resident grid points stay under ``max_cells``
+(enough to keep the device saturated; beyond it every resident job would
+only slow the others down).  A job that does not fit may be overtaken by a
+later one that does (backfill).  Admitted jobs stay resident in one
+``Batch`` per grid shape, each advanced by its own thread on its own
+streams, so small jobs do not wait for the step of a large one; jobs join
+and leave a batch at step boundaries, every ``chunk`` steps.  Host fields
+are prepared on worker threads while the device computes, and each job's
+time to result (arrival to final step) is recorded.
+"""
+import concurrent.futures as cf
+import math
+import threading
+import time
+
+import numpy as np
+
+from . import configs
+
+SWEEP_GRIDS = [(64, 64, 65), (128, 128, 97), (192, 128, 129), (256, 256, 193)]
+SERVED_GRIDS = SWEEP_GRIDS + [(384, 384, 289)]
+
+
+def _instance(rng, grid, steps):
+    nthm, nzm, nrm = grid
+    eta = rng.uniform(0.5, 0.909)
+    ren = 10.0 ** rng.uniform(3.0, 4.0)
+    ratio = rng.uniform(-0.5, 0.2)
+    lamb = float(rng.choice([2.0, 4.0, 8.0]))
+    p = dict(configs.BASE)
+    p.update(nthm=nthm, nzm=nzm, nrm=nrm, idtv=0, ntst=steps - 1, alx3=2.0 * math.pi,
+             rint=1.0, rext=1.0 / eta, lamb=lamb, virotint=1.0, virotext=ratio,
+             ren=ren, limitcfl=1.0e30)
+    return p
+
+
+def sweep_jobs(n=48, seed=4004, steps=300):
+    """(params, field seed) of the batched sweep; the fixed dt is set by
+    ``start_fields``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        grid = SWEEP_GRIDS[int(rng.integers(len(SWEEP_GRIDS)))]
+        out.append((_instance(rng, grid, steps), seed * 1000 + q))
+    return out
+
+
+def served_jobs(n, seed=5005, rate=2.0, steps=1000):
+    """(arrival time in s, params, field seed) of the served sweep."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for q in range(n):
+        t += rng.exponential(1.0 / rate)
+        grid = SERVED_GRIDS[int(rng.integers(len(SERVED_GRIDS)))]
+        p = _instance(rng, grid, steps)
+        p.update(sizepert=rng.uniform(0.0, 0.2), omegapert=rng.uniform(0.01, 0.1))
+        out.append((t, p, seed * 1000 + q))
+    return out
+
+
+def cfl_estimate(tc, p, qt, qr, qz):
+    """CalcMaxCFL of a host field (Fortran layout (nr, nzm, nthm))."""
+    nrm = p["nrm"]
+    usrm = tc.grid_vector(p, "usrm")[:nrm]
+    udx2rm = tc.grid_vector(p, "udx2rm")[:nrm]
+    dxt = p["nthm"] * p["lamb"] / (2.0 * math.pi)
+    dxz = p["nzm"] / p["alx3"]
+    t = qt[:nrm]
+    a = np.abs((np.roll(t, -1, axis=2) + t) * 0.5 * dxt * usrm[:, None, None])
+    a += np.abs((qr[1:nrm + 1] + qr[:nrm]) * 0.5 * udx2rm[:, None, None])
+    z = qz[:nrm]
+    a += np.abs((np.roll(z, -1, axis=1) + z) * 0.5 * dxz)
+    return max(1e-8, float(a.max()))
+
+
+def pressure_step_amplification(tc, p, dt):
+    """Largest per-step growth factor of a pressure error mode under the
+    reference's RK3 step.
+
+    CorrectPressure adds phi - beta L phi with beta = dt/(2 Re) (not
+    al*beta), and the implicit part of each substep is radial only.  A
+    pressure error mode with radial eigenvalue part x/beta and theta-z part
+    y/beta is multiplied in substep s by
+    -((1 - al_s) x + y) / (1 + al_s x); the step multiplies the three.  The
+    bound takes y at its largest (theta and z Nyquist at the inner radius)
+    and x over [0, Gershgorin bound of the radial operator]."""
+    nrm = p["nrm"]
+    acph = tc.grid_vector(p, "acph")[:nrm]
+    usrm = tc.grid_vector(p, "usrm")[:nrm]
+    dxt = p["nthm"] * p["lamb"] / (2.0 * math.pi)
+    dxz = p["nzm"] / p["alx3"]
+    beta = dt / (2.0 * p["ren"])
+    y = beta * (4.0 * dxt * dxt * float(np.max(usrm * usrm)) + 4.0 * dxz * dxz)
+    x = np.linspace(0.0, beta * 2.0 * float(np.max(np.abs(acph))), 2001)
+    g = np.ones_like(x)
+    for al in (8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0):
+        g = g * ((1.0 - al) * x + y) / (1.0 + al * x)
+    return float(g.max())
+
+
+def pressure_stable_dt(tc, p, dt_hi, safety=0.9):
+    """dt_hi if the pressure update is stable there, else `safety` times the
+    largest dt below it with a step amplification of at most 1."""
+    if pressure_step_amplification(tc, p, dt_hi) <= 1.0:
+        return dt_hi
+    lo, hi = 0.0, dt_hi
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if pressure_step_amplification(tc, p, mid) <= 1.0:
+            lo = mid
+        else:
+            hi = mid
+    return safety * lo
+
+
+def start_fields(tc, p, seed, amp=1e-2, cfl=0.8):
+    """Seeded perturbed Couette start and its fixed dt: the CFL = 0.8
+    estimate of the start, reduced by ``pressure_stable_dt`` where the
+    reference's pressure update would amplify errors; sets
+    p['dtmax'] = p['dtmin'] = dt in place."""
+    qt, qr, qz = tc.init_perturbed_couette(p, seed, amp)
+    dt = pressure_stable_dt(tc, p, cfl / cfl_estimate(tc, p, qt, qr, qz))
+    p["dtmax"] = p["dtmin"] = dt
+    return qt, qr, qz
+
+
+def footprint_bytes(p):
+    """Device bytes of one instance: eleven fields of nr planes, the
+    spectrum, and the small per-instance tables."""
+    nthm, nzm, nrm = p["nthm"], p["nzm"], p["nrm"]
+    sstr = ((nthm // 2 + 1) + 7) // 8 * 8
+    return 8 * 11 * (nrm + 1) * nzm * nthm + 16 * nrm * nzm * sstr + (8 << 20)
+
+
+def load_and_project(solver, fields):
+    for name, a in zip(("qt", "qr", "qz"), fields):
+        solver.set(name, a)
+    solver.project()
+
+
+class _ClassWorker:
+    """One Batch of one grid shape, advanced by its own thread: jobs of a
+    small shape are not held to the step pace of a large one (their kernels
+    interleave with the large shape's on the device).  New jobs are handed
+    over through a queue and join the batch at the next chunk boundary."""
+
+    def __init__(self, server, shape):
+        self.srv = server
+        self.shape = shape
+        self.b = server.tc.Batch()
+        self.qlock = threading.Lock()
+        self.pending = []          # (job index, params, fields, tuning)
+        self.active = {}           # batch id -> job index
+        self.wake = threading.Event()
+        self.stop = False
+        self.error = None
+        self.thread = threading.Thread(target=self._loop, daemon=True)
+        self.thread.start()
+
+    def admit(self, q, p, fields, tuning):
+        with self.qlock:
+            self.pending.append((q, p, fields, tuning))
+        self.wake.set()
+
+    def _loop(self):
+        srv, b = self.srv, self.b
+        try:
+            while True:
+                if not self.active:
+                    self.wake.wait(0.05)
+                self.wake.clear()
+                if self.stop:
+                    break
+                with self.qlock:
+                    new, self.pending = self.pending, []
+                for q, p, fields, tuning in new:
+                    iid = b.add(p, tuning)
+                    load_and_project(b.solver(iid), fields)
+                    self.active[iid] = q
+                    srv._admitted(q)
+                del new
+                if not self.active:
+                    continue
+                b.advance(srv.chunk, graph=srv.graph, sync_every=srv.chunk)
+                for iid in list(self.active):
+                    steps, code, _ = b.status(iid)
+                    if code:
+                        q = self.active.pop(iid)
+                        fields = None
+                        if srv.keep_fields:
+                            s = b.solver(iid)
+                            fields = tuple(s.get(f) for f in ("qt", "qr", "qz", "pr"))
+                        b.remove(iid)
+                        srv._finished(q, steps, code, fields)
+        except Exception as e:  # reported by Server.run
+            self.error = e
+            srv._finished(None, 0, 0, None)
+
+    def close(self):
+        self.stop = True
+        self.wake.set()
+        self.thread.join()
+        self.b.close()
+
+
+class Server:
+    """Continuous-batching request loop: one resident Batch per grid shape,
+    each advanced by its own thread; admission is global."""
+
+    def __init__(self, tc, hbm_budget=200e9, chunk=25, graph=False, tuning=None, workers=4,
+                 max_slots=16, max_cells=400e6):
+        # graph=False: the class workers run on their own threads, and HIP
+        # refuses legacy-stream copies (field uploads) while another thread
+        # captures a graph; eager grouped launches measure the same here
+        self.tc = tc
+        self.budget = hbm_budget
+        self.max_slots = max_slots
+        self.max_cells = max_cells
+        self.chunk = chunk
+        self.graph = graph
+        self.tuning = tuning or {}
+        self.pool = cf.ThreadPoolExecutor(max_workers=workers)
+        self.ahead = workers + 2
+        self.cv = threading.Condition()
+
+    def _admitted(self, q):
+        with self.cv:
+            self.rec[q]["admitted"] = time.perf_counter() - self.t0
+
+    def _finished(self, q, steps, code, fields):
+        with self.cv:
+            if q is not None:
+                r = self.rec[q]
+                r.update(done=time.perf_counter() - self.t0, steps=steps, code=code)
+                if fields is not None:
+                    r["fields"] = fields
+                self.used -= footprint_bytes(self.jobs[q][1])
+                self.cells -= r["cells"]
+                self.nactive -= 1
+            self.cv.notify_all()
+
+    def run(self, jobs, keep_fields=False):
+        """jobs: (arrival s, params, seed).  Returns one record per job:
+        arrival, admitted, done (s since start), steps, code, cells; with
+        keep_fields, the final (qt, qr, qz, pr)."""
+        self.jobs, self.keep_fields = jobs, keep_fields
+        self.rec = [None] * len(jobs)
+        self.used, self.cells, self.nactive = 0.0, 0, 0
+        self.t0 = time.perf_counter()
+        workers = {}
+        waiting = sorted(range(len(jobs)), key=lambda q: jobs[q][0])
+        futures = {}
+        try:
+            while True:
+                with self.cv:
+                    for w in workers.values():
+                        if w.error:
+                            raise w.error
+                    if not waiting and self.nactive == 0:
+                        break
+                for q in waiting[: self.ahead]:
+                    if q not in futures:
+                        futures[q] = self.pool.submit(self._prepare, jobs[q][1], jobs[q][2])
+                now = time.perf_counter() - self.t0
+                # admission: arrived jobs in arrival order, each if a slot,
+                # the HBM budget and the resident-cell target allow (always
+                # at least one resident job); later jobs may backfill past
+                # one that waits
+                for q in [q for q in waiting if jobs[q][0] <= now]:
+                    p0 = jobs[q][1]
+                    need = footprint_bytes(p0)
+                    c = p0["nthm"] * p0["nzm"] * p0["nrm"]
+                    with self.cv:
+                        if self.nactive >= self.max_slots:
+                            break
+                        if self.nactive and (self.used + need > self.budget or
+                                             self.cells + c > self.max_cells):
+                            continue
+                        self.used += need
+                        self.cells += c
+                        self.nactive += 1
+                    if q not in futures:
+                        futures[q] = self.pool.submit(self._prepare, p0, jobs[q][2])
+                    waiting.remove(q)
+                    p, fields = futures.pop(q).result()
+                    sh = (p["nthm"], p["nzm"], p["nrm"])
+                    if sh not in workers:
+                        workers[sh] = _ClassWorker(self, sh)
+                    with self.cv:
+                        self.rec[q] = dict(arrival=jobs[q][0], admitted=None, cells=c,
+                                           steps_total=p["ntst"] + 1)
+                    workers[sh].admit(q, p, fields, self.tuning.get(sh))
+                    del fields
+                # sleep until a job finishes or the next arrival
+                with self.cv:
+                    nxt = min((jobs[q][0] for q in waiting), default=None)
+                    timeout = 0.5 if nxt is None else max(0.0, nxt - (time.perf_counter() - self.t0))
+                    self.cv.wait(min(timeout, 0.5))
+        finally:
+            for w in workers.values():
+                w.close()
+        return self.rec
+
+    def _prepare(self, p, seed):
+        p = dict(p)
+        fields = start_fields(self.tc, p, seed)
+        return p, fields

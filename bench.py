@@ -32,15 +32,16 @@ PROFILES = os.path.join(ROOT, "profiles")
 
 # rocprofv3 commands whose output --summarize-profile turns into the committed
 # summaries (run from /tmp with TMPDIR=/tmp; OUT is the output folder)
+_LEG = "--no-cpu --sweep-steps 0 --served-jobs 0"
 PROFILE_COMMANDS = {
     "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o bench "
-             "-- python3 bench.py --steps 5 --warmup 1 --no-cpu",
+             f"-- python3 bench.py --steps 5 --warmup 1 {_LEG}",
     "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE --output-format csv --kernel-include-regex "
-                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_divergence|k_diagnostics\" "
-                  "-d OUT/FETCH_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 --no-cpu",
+                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
+                  f"-d OUT/FETCH_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 {_LEG}",
     "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE --output-format csv --kernel-include-regex "
-                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_divergence|k_diagnostics\" "
-                  "-d OUT/WRITE_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 --no-cpu",
+                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
+                  f"-d OUT/WRITE_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 {_LEG}",
 }
 
 

@@ -87,7 +87,7 @@ typedef struct tc_tuning {
   int fft_z_lds;  /* LDS bytes of a z-FFT block (default 65536) */
   int fft_z_nt;   /* threads of a z-FFT block: 256 or 512 (default) */
   int fft_pad;    /* LDS padding of the FFT tiles: 1 on (default), -1 off */
-  int poisson_pq; /* rows per thread of the radial Poisson solve: 13 or 26 (default) */
+  int poisson_pq; /* rows per thread of the radial Poisson solve: 13 (default) or 26 */
 } tc_tuning;
 
 /* tc_create with an explicit schedule; tc_set_tuning re-plans an existing

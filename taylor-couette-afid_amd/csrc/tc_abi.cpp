@@ -283,7 +283,7 @@ tc_tuning resolve_tuning(const tc_tuning *in) {
   if (t.fft_z_lds == 0) t.fft_z_lds = env("TC_FFT_Z_LDS", 65536);
   if (t.fft_z_nt == 0) t.fft_z_nt = env("TC_FFT_Z_NT", 512) == 256 ? 256 : 512;
   if (t.fft_pad == 0) t.fft_pad = env("TC_FFT_PAD", 1) == 0 ? -1 : 1;
-  if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", 26) == 13 ? 13 : 26;
+  if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", 13) == 26 ? 26 : 13;
   return t;
 }
 
@@ -504,7 +504,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   c->poisson_pq = t.poisson_pq;
   {
     const char *dfs = std::getenv("TC_DIVFFT_STAGE");
-    c->divfft_stage = dfs ? std::atoi(dfs) : 0;
+    c->divfft_stage = dfs ? std::atoi(dfs) : 1;
   }
   TRY(refresh_tables(c));
   HTRY(hipStreamSynchronize(c->stream));

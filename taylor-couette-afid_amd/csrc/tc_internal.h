@@ -125,8 +125,8 @@ struct tc_ctx {
   double time = 0, dt = 0, beta = 0, al = 0, ga = 0, ro = 0, vinner_t = 0, dvinner_t = 0;
   int ns = 1;
   bool fused = true;
-  int divfft_stage = 0;  // fused divergence FFT: stage the divergence rows in LDS first
-  int poisson_pq = 26;  // rows per thread of the radial Poisson solve (13 or 26)
+  int divfft_stage = 1;  // fused divergence FFT: stage the divergence rows in LDS first
+  int poisson_pq = 13;  // rows per thread of the radial Poisson solve (13 or 26)
   int helm_lpt = 1;  // radial lines per thread in the fused Helmholtz kernel (1 or 2)
   tc_tuning tuning{};  // effective tuning (helm_lpt 0 = chosen by occupancy)
   // CFL of the current velocity, when known (set by the logging pass of tc_step)

@@ -14,6 +14,7 @@
 // rounding.  The FFTs are Stockham autosort passes in LDS with radix 8/4/2/3/5
 // butterflies and twiddles from a host-computed table.
 #include <cmath>
+#include <cstdlib>
 
 #include "tc_internal.h"
 
@@ -113,32 +114,66 @@ DEVI void dft<8>(double2 *v, double sg) {
 // Stockham writes over the LDS banks.  ME bounds the points a thread holds per pass
 // (n <= ME*T), so the register arrays have a compile-time size.  All threads
 // of the block must call.
-template <int R, int ME, bool FROM_G, bool TO_G, class Load, class Store>
+// BATCH: all loads first and unconditionally (a thread past the last
+// butterfly repeats the last one and does not store it), so they are issued
+// as one batch, at the cost of registers; tw[0] = 1 exactly, so k = 0 needs
+// no branch.  Otherwise each butterfly loads and multiplies in turn.
+template <int R, int ME, bool FROM_G, bool TO_G, bool BATCH, class Load, class Store>
 DEVI void fft_pass(double2 *buf, int off, int stride, int sh, int n, int Ns, int t, int T,
                    const double2 *__restrict__ tw, double sg, Load &ld, Store &st) {
   constexpr int MB = (ME + R - 1) / R;
   const int nb = n / R;
   const int twstep = n / (Ns * R);
   double2 v[MB][R];
+  if constexpr (!BATCH) {
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+      const int b = t + q * T;
+      if (b < nb) {
+        const int k = b % Ns;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          double2 x;
+          if constexpr (FROM_G) x = ld(b + r * nb);
+          else x = buf[pidx(off + (b + r * nb) * stride, sh)];
+          if (r > 0 && k > 0) {
+            double2 w = tw[r * k * twstep];
+            w.y = (sg < 0) ? w.y : -w.y;
+            x = cmul(x, w);
+          }
+          v[q][r] = x;
+        }
+        dft<R>(v[q], sg);
+      }
+    }
+  } else {
 #pragma unroll
   for (int q = 0; q < MB; q++) {
-    const int b = t + q * T;
-    if (b < nb) {
-      const int k = b % Ns;
+    const int b = min(t + q * T, nb - 1);
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        double2 x;
-        if constexpr (FROM_G) x = ld(b + r * nb);
-        else x = buf[pidx(off + (b + r * nb) * stride, sh)];
-        if (r > 0 && k > 0) {
-          double2 w = tw[r * k * twstep];
-          w.y = (sg < 0) ? w.y : -w.y;
-          x = cmul(x, w);
-        }
-        v[q][r] = x;
-      }
-      dft<R>(v[q], sg);
+    for (int r = 0; r < R; r++) {
+      if constexpr (FROM_G) v[q][r] = ld(b + r * nb);
+      else v[q][r] = buf[pidx(off + (b + r * nb) * stride, sh)];
     }
+  }
+  double2 w[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; q++) {
+    const int b = min(t + q * T, nb - 1);
+    const int k = b % Ns;
+#pragma unroll
+    for (int r = 1; r < R; r++) w[q][r] = tw[r * k * twstep];
+  }
+#pragma unroll
+  for (int q = 0; q < MB; q++) {
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      double2 x = w[q][r];
+      x.y = (sg < 0) ? x.y : -x.y;
+      v[q][r] = cmul(v[q][r], x);
+    }
+    dft<R>(v[q], sg);
+  }
   }
   if constexpr (!FROM_G) __syncthreads();
 #pragma unroll
@@ -157,36 +192,38 @@ DEVI void fft_pass(double2 *buf, int off, int stride, int sh, int n, int Ns, int
   if constexpr (!TO_G) __syncthreads();
 }
 
-template <int ME, bool FROM_G, bool TO_G, class Load, class Store>
+template <int ME, bool FROM_G, bool TO_G, bool BATCH, class Load, class Store>
 DEVI void fft_pass_r(int R, double2 *buf, int off, int stride, int sh, int n, int Ns, int t,
                      int T, const double2 *__restrict__ tw, double sg, Load &ld, Store &st) {
+#define TC_FP(RR) fft_pass<RR, ME, FROM_G, TO_G, BATCH>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st)
   switch (R) {
-    case 8: fft_pass<8, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
-    case 4: fft_pass<4, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
-    case 2: fft_pass<2, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
-    case 3: fft_pass<3, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
-    default: fft_pass<5, ME, FROM_G, TO_G>(buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st); break;
+    case 8: TC_FP(8); break;
+    case 4: TC_FP(4); break;
+    case 2: TC_FP(2); break;
+    case 3: TC_FP(3); break;
+    default: TC_FP(5); break;
   }
+#undef TC_FP
 }
 
 // A whole line: with FROM_G the input comes from ld(e), otherwise from LDS;
 // with TO_G the output goes to st(e, value), otherwise to LDS (in natural
 // order, followed by a barrier).
-template <int ME, bool FROM_G, bool TO_G, class Load, class Store>
+template <int ME, bool FROM_G, bool TO_G, bool BATCH = true, class Load, class Store>
 DEVI void fft_line(double2 *buf, int off, int stride, const FFTPlan &pl, int t, int T,
                    const double2 *__restrict__ tw, double sg, Load ld, Store st) {
   const int n = pl.n, ns = pl.nst, sh = pl.padsh;
   if (ns == 1) {
-    fft_pass_r<ME, FROM_G, TO_G>(pl.radix[0], buf, off, stride, sh, n, 1, t, T, tw, sg, ld, st);
+    fft_pass_r<ME, FROM_G, TO_G, BATCH>(pl.radix[0], buf, off, stride, sh, n, 1, t, T, tw, sg, ld, st);
     return;
   }
-  fft_pass_r<ME, FROM_G, false>(pl.radix[0], buf, off, stride, sh, n, 1, t, T, tw, sg, ld, st);
+  fft_pass_r<ME, FROM_G, false, BATCH>(pl.radix[0], buf, off, stride, sh, n, 1, t, T, tw, sg, ld, st);
   int Ns = pl.radix[0];
   for (int s = 1; s < ns - 1; s++) {
-    fft_pass_r<ME, false, false>(pl.radix[s], buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st);
+    fft_pass_r<ME, false, false, BATCH>(pl.radix[s], buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st);
     Ns *= pl.radix[s];
   }
-  fft_pass_r<ME, false, TO_G>(pl.radix[ns - 1], buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st);
+  fft_pass_r<ME, false, TO_G, BATCH>(pl.radix[ns - 1], buf, off, stride, sh, n, Ns, t, T, tw, sg, ld, st);
 }
 
 // r2c along theta for row pairs: rows 2p and 2p+1 of the real input become
@@ -288,8 +325,8 @@ template <int ME>
 __global__ __launch_bounds__(256) void k_div_fft_theta_fwd_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
   const DivIn dv{d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.usdt};
-  theta_fwd_body<ME, true, false>(blockIdx.x, nullptr, dv, d.g, d.spec, d.pt, d.twt,
-                                  (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
+  theta_fwd_body<ME, true, true>(blockIdx.x, nullptr, dv, d.g, d.spec, d.pt, d.twt,
+                                 (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
 }
 
 // c2r along theta for row pairs; the imaginary parts of ktheta = 0 and
@@ -309,15 +346,30 @@ DEVI void theta_inv_body(const long long pb, const double2 *__restrict__ spec,
   const bool v0 = r0 < nrows, v1 = r1 < nrows;
   double2 *buf = lds;
   const int off = ln * n, sh = pl.padsh;
-  for (int k = t; k < nth2; k += T) {
-    double2 A = v0 ? spec[r0 * sstr + k] : make_double2(0.0, 0.0);
-    double2 B = v1 ? spec[r1 * sstr + k] : make_double2(0.0, 0.0);
-    if (k == 0 || 2 * k == n) {
-      A.y = 0.0;
-      B.y = 0.0;
+  // T * ME >= n, so nth2 <= T * (ME/2 + 1): the loads of all rounds are
+  // issued before any is used (clamped rows and columns stay in bounds)
+  constexpr int KR = ME / 2 + 1;
+  double2 As[KR], Bs[KR];
+  const long long ra = v0 ? r0 : 0, rb = v1 ? r1 : 0;
+#pragma unroll
+  for (int u = 0; u < KR; u++) {
+    const int k = min(t + u * T, nth2 - 1);
+    As[u] = spec[ra * sstr + k];
+    Bs[u] = spec[rb * sstr + k];
+  }
+#pragma unroll
+  for (int u = 0; u < KR; u++) {
+    const int k = t + u * T;
+    if (k < nth2) {
+      double2 A = v0 ? As[u] : make_double2(0.0, 0.0);
+      double2 B = v1 ? Bs[u] : make_double2(0.0, 0.0);
+      if (k == 0 || 2 * k == n) {
+        A.y = 0.0;
+        B.y = 0.0;
+      }
+      buf[pidx(off + k, sh)] = make_double2(A.x - B.y, A.y + B.x);
+      if (k > 0 && 2 * k != n) buf[pidx(off + n - k, sh)] = make_double2(A.x + B.y, B.x - A.y);
     }
-    buf[pidx(off + k, sh)] = make_double2(A.x - B.y, A.y + B.x);
-    if (k > 0 && 2 * k != n) buf[pidx(off + n - k, sh)] = make_double2(A.x + B.y, B.x - A.y);
   }
   __syncthreads();
   auto ld = [&](int) { return make_double2(0.0, 0.0); };
@@ -370,7 +422,7 @@ DEVI void fft_z_body(const int id, double2 *__restrict__ spec, const FFTPlan &pl
   auto st = [&](int z, double2 v) {
     if (ok) col[(long long)z * sstr] = v;
   };
-  fft_line<ME, true, true>(lds, c, L, pl, t, T, tw, sg, ld, st);
+  fft_line<ME, true, true, false>(lds, c, L, pl, t, T, tw, sg, ld, st);
 }
 
 template <int ME, int NT>
@@ -407,6 +459,7 @@ __global__ __launch_bounds__(NT) void k_fft_z_g(const StepDesc *__restrict__ tab
 // one line in LDS and solves it with a serial Thomas sweep instead.
 //   blockDim = (LB lines, PT partitions)
 //   LDS = LB*2PT*4 doubles (reduced systems) + 3*nrm doubles (gauge line)
+//         + 4*nrm doubles (the radial coefficients amph, apph, acph, usrm^2)
 template <int PQ>  // max rows per thread
 DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restrict__ spec,
                        double inv_norm) {
@@ -419,12 +472,20 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
   const int n = g.nrm;
   double2 *gline = (double2 *)(red + (size_t)LB * 2 * PT * 4);  // gauge line, n values
   double *gcp = (double *)(gline + n);                          // its multipliers
+  double *cam = gcp + n, *cap = cam + n, *cac = cap + n, *cus = cac + n;  // row k at [k-1]
   const bool gblock = (blk == 0);
   if (gblock)
     for (int k = tid; k < n; k += nthr) {
       const double2 v = spec[(long long)k * M];
       gline[k] = make_double2(v.x * inv_norm, v.y * inv_norm);
     }
+  // the line-independent radial coefficients, staged once per block
+  for (int k = tid; k < n; k += nthr) {
+    cam[k] = g.amph[k + 1];
+    cap[k] = g.apph[k + 1];
+    cac[k] = g.acph[k + 1];
+    cus[k] = g.usrm[k + 1] * g.usrm[k + 1];
+  }
   bool valid = m < M && m != 0;
   int kz = 0, kt = 0;
   if (valid) {
@@ -436,21 +497,28 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
   const int lo = (int)(((long long)pp * n) / PT), S = (int)(((long long)(pp + 1) * n) / PT) - lo;
   // row k (1-based) of this line: a x(k-1) + b x(k) + c x(k+1) = d
   auto coef = [&](int k, double &a, double &b, double &c) {
-    a = (k > 1) ? g.amph[k] : 0.0;
-    c = (k < n) ? g.apph[k] : 0.0;
-    b = g.acph[k] - sz - st * (g.usrm[k] * g.usrm[k]);
+    a = (k > 1) ? cam[k - 1] : 0.0;
+    c = (k < n) ? cap[k - 1] : 0.0;
+    b = cac[k - 1] - sz - st * cus[k - 1];
     if (!valid) { a = 0.0; b = 1.0; c = 0.0; }
   };
   double ar[PQ], cr[PQ];
   double2 d[PQ];
+  // unconditional loads (clamped row, line 0 for idle lanes) issued as one
+  // batch; rows past the partition and idle lanes are zeroed after
+  {
+    const long long mm = valid ? m : 0;
 #pragma unroll
-  for (int i = 0; i < PQ; i++) {
-    d[i] = make_double2(0.0, 0.0);
-    if (i < S && valid) {
-      const double2 v = spec[(long long)(lo + i) * M + m];
-      d[i] = make_double2(v.x * inv_norm, v.y * inv_norm);
+    for (int i = 0; i < PQ; i++) {
+      const int row = min(lo + i, n - 1);
+      d[i] = spec[(long long)row * M + mm];
     }
+#pragma unroll
+    for (int i = 0; i < PQ; i++)
+      d[i] = (i < S && valid) ? make_double2(d[i].x * inv_norm, d[i].y * inv_norm)
+                              : make_double2(0.0, 0.0);
   }
+  __syncthreads();  // coefficient tables staged
   // downward: rows i >= 1 become a_i x_0 + x_i + c_i x_(i+1) = d_i
 #pragma unroll
   for (int i = 0; i < PQ; i++) {
@@ -597,7 +665,11 @@ void poisson_part_shape(int n, int PQ, int &pt, int &lb) {
   pt = (n + PQ - 1) / PQ;
   if (pt < 1) pt = 1;
   while (pt > 1 && n < 2 * pt) pt--;
-  const int maxt = (PQ <= 13) ? 256 : 512;
+  static const int env_maxt = [] {
+    const char *e = std::getenv("TC_POISSON_MAXT");
+    return e ? std::atoi(e) : 0;
+  }();
+  const int maxt = env_maxt > 0 ? env_maxt : ((PQ <= 13) ? 256 : 512);
   lb = 64;
   while (lb > 1 && lb * pt > maxt) lb /= 2;
 }
@@ -727,7 +799,7 @@ void launch_poisson(tc_ctx *c, bool with_divergence, double usdt) {
     int pt, lb;
     const int pq = c->poisson_pq;
     poisson_part_shape(c->nrm, pq, pt, lb);
-    const size_t lds = sizeof(double) * (4 * 2 * (size_t)pt * lb + 3 * (size_t)c->nrm);
+    const size_t lds = sizeof(double) * (4 * 2 * (size_t)pt * lb + 7 * (size_t)c->nrm);
     const dim3 grid((unsigned)((M + lb - 1) / lb)), block(lb, pt);
     const double inv_norm = 1.0 / ((double)c->nthm * (double)c->nzm);
     if (pq == 13)
@@ -765,7 +837,7 @@ void launch_poisson_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t st) {
     if (pass == 1) {
       int ptn, lb;
       poisson_part_shape(c->nrm, c->poisson_pq, ptn, lb);
-      const size_t lds = sizeof(double) * (4 * 2 * (size_t)ptn * lb + 3 * (size_t)c->nrm);
+      const size_t lds = sizeof(double) * (4 * 2 * (size_t)ptn * lb + 7 * (size_t)c->nrm);
       const long long M = (long long)c->nzm * c->sstr;
       const dim3 grid((unsigned)((M + lb - 1) / lb), (unsigned)n), block(lb, ptn);
       if (c->poisson_pq == 13)

@@ -1,7 +1,7 @@
 """ctypes view of liboracle.so, the CPU restatement of the reference time step.
 
 Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  Fields are numpy arrays of shape
+cpu_baseline leg of bench.py, as the reference result.  Fields are numpy arrays of shape
 (nr, nzm, nthm) in Fortran order, i.e. the reference's (1:nr,1:nzm,1:nthm)
 without halos.
 """

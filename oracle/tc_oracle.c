@@ -1,7 +1,7 @@
 /*
  * tc_oracle.c -- CPU restatement of the Taylor-Couette AFiD RK3 time step.
  *
- * TEST INFRASTRUCTURE ONLY: the parity checker for the MI355X path, never
+ * TEST INFRASTRUCTURE ONLY: the parity reference for the MI355X path, never
  * part of it.  Loaded only by tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py.
  *

@@ -1,7 +1,7 @@
 /*
  * tc_oracle.h -- CPU restatement of the Taylor-Couette AFiD RK3 time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference for the
  * MI355X implementation.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it; the product path never does.
  *

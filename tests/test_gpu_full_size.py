@@ -94,3 +94,35 @@ def test_projection_idempotent_full_size(tc):
         e = np.abs(dev.get(f) - a[f]).max() / np.abs(a["qt"]).max()
         assert e <= 1e-13, (f, e)
     dev.close()
+
+
+def _stats(dev):
+    vmax, rms = dev.CalcVelocityStats()
+    t_in, t_out = dev.CalcTorque()
+    return np.concatenate([vmax, rms, [t_in, t_out, dev.CalcMaxCFL()]])
+
+
+def test_large_1024_fused_and_routine_paths_agree(tc):
+    """1024 x 1024 x 769 (eta 0.909, counter-rotating outer cylinder), the
+    largest workload: projection to roundoff, two RK3 steps below RESID on
+    each kernel set, and the two kernel sets agree on the velocity maxima,
+    mean squares, wall torques and CFL (device reductions, so the 6.5 GB
+    fields never leave the device)."""
+    p = tc.configs.large_1024()
+    print("1024x1024x769: fused vs routine-by-routine, 2 steps", flush=True)
+    qt, qr, qz = tc.init_perturbed_couette(p, tc.configs.SEEDS["large_1024"],
+                                           tc.configs.AMPLITUDE["large_1024"])
+    out = []
+    for fused in (True, False):
+        dev = tc.Solver(p)
+        dev.set_fused(fused)
+        dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+        dev.project()
+        assert dev.CheckDivergence() < 1e-10
+        for n in range(2):
+            code, d = dev.step(n)
+            assert code == 0 and d["dmax"] < p["resid"], (fused, n, code, d)
+        out.append(_stats(dev))
+        print(f"  fused={fused}: {out[-1]}", flush=True)
+        dev.close()
+    assert np.allclose(out[0], out[1], rtol=1e-11, atol=0), (out[0], out[1])

@@ -194,6 +194,20 @@ int tc_copy_field(tc_ctx *ctx, int dst, int src);
 /* Device bytes held by the context (fields + scratch). */
 int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
 
+/* ---------------- checkpoint / restart ---------------- */
+
+/* Whole-state checkpoint in one raw file: WriteFlowField.F90:11-53 writes
+ * qt, qr, qz and the master scalars (time, grid sizes and geometry); this
+ * also keeps pr, the RK histories ru1..ru3, ntime and the CFL of the last
+ * logging pass, so that a restarted run continues bit for bit (the
+ * reference's restart, ReadFlowField.F90:135-144, restarts with pr = 0).
+ * Layout: 8-byte magic "TCAFIDR1", int32 nthm nzm nrm nsst ntime cfl_valid
+ * istr pad, float64 time dt cfl rint rext alx3 lamb str, then the fields
+ * qt qr qz pr ru1 ru2 ru3 in device order ([r][z][theta], nr planes each).
+ * tc_read_restart rejects a file of another grid or geometry. */
+int tc_write_restart(tc_ctx *ctx, const char *path, int ntime);
+int tc_read_restart(tc_ctx *ctx, const char *path, int *ntime);
+
 /* ---------------- batched sweep (no counterpart in the reference) ---------------- */
 
 /* Many independent instances, each a tc_ctx of its own shape and

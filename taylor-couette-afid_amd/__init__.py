@@ -60,6 +60,7 @@ ABI_SYMBOLS = [
     "tc_create_tuned", "tc_set_tuning", "tc_get_tuning", "tc_autotune",
     "tc_batch_create", "tc_batch_destroy", "tc_batch_add", "tc_batch_remove", "tc_batch_ctx",
     "tc_batch_advance", "tc_batch_status", "tc_batch_groups",
+    "tc_write_restart", "tc_read_restart",
 ]
 
 TUNING_FIELDS = ["helm_lpt", "fft_t_lds", "fft_z_lds", "fft_z_nt", "fft_pad", "poisson_pq"]
@@ -153,6 +154,8 @@ def lib():
     L.tc_batch_advance.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.tc_batch_status.argtypes = [_V, ctypes.c_int, _I, _I, _D]
     L.tc_batch_groups.argtypes = [_V, _I, _I]
+    L.tc_write_restart.argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
+    L.tc_read_restart.argtypes = [_V, ctypes.c_char_p, _I]
     _LIB = L
     return L
 
@@ -330,6 +333,16 @@ class Solver:
 
     def project(self):
         _check(lib().tc_project(self.h))
+
+    def write_restart(self, path, ntime):
+        """Whole-state checkpoint (fields, pressure, RK histories, time)."""
+        _check(lib().tc_write_restart(self.h, os.fsencode(path), int(ntime)))
+
+    def read_restart(self, path):
+        """Loads a checkpoint; returns the ntime it was written at."""
+        n = ctypes.c_int()
+        _check(lib().tc_read_restart(self.h, os.fsencode(path), ctypes.byref(n)))
+        return n.value
 
     def step(self, ntime, walltime=0.0):
         """One main.F90 loop iteration; returns (errorcode, diagnostics)."""

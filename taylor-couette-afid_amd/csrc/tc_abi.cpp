@@ -6,6 +6,7 @@
 #include <cmath>
 #include <mutex>
 #include <tuple>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -621,6 +622,88 @@ int tc_autotune(const tc_params *p, int steps, tc_tuning *best) {
     cache[key] = cur;
   }
   *best = cur;
+  return 0;
+}
+
+namespace {
+struct RestartHeader {
+  char magic[8];
+  int32_t nthm, nzm, nrm, nsst, ntime, cfl_valid, istr, pad;
+  double time, dt, cfl, rint, rext, alx3, lamb, str;
+};
+const int kRestartFields[] = {TC_QT, TC_QR, TC_QZ, TC_PR, TC_RU1, TC_RU2, TC_RU3};
+}  // namespace
+
+int tc_write_restart(tc_ctx *c, const char *path, int ntime) {
+  CTXCHK(c);
+  if (!path) return fail("null path");
+  RestartHeader h;
+  std::memset(&h, 0, sizeof h);
+  std::memcpy(h.magic, "TCAFIDR1", 8);
+  h.nthm = c->nthm;
+  h.nzm = c->nzm;
+  h.nrm = c->nrm;
+  h.nsst = c->p.nsst;
+  h.ntime = ntime;
+  h.cfl_valid = c->cfl_valid ? 1 : 0;
+  h.istr = c->p.istr;
+  h.time = c->time;
+  h.dt = c->dt;
+  h.cfl = c->cfl_cached;
+  h.rint = c->p.rint;
+  h.rext = c->p.rext;
+  h.alx3 = c->p.alx3;
+  h.lamb = c->p.lamb;
+  h.str = c->p.str;
+  std::FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(std::string("cannot open ") + path + " for writing");
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  std::vector<double> buf((size_t)c->N);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q : kRestartFields) {
+    if (!ok) break;
+    HIPCHK(hipMemcpyAsync(buf.data(), c->f[q], sizeof(double) * buf.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ok = std::fwrite(buf.data(), sizeof(double), buf.size(), f) == buf.size();
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? 0 : fail(std::string("short write to ") + path);
+}
+
+int tc_read_restart(tc_ctx *c, const char *path, int *ntime) {
+  CTXCHK(c);
+  if (!path) return fail("null path");
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(std::string("cannot open ") + path);
+  RestartHeader h;
+  if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, "TCAFIDR1", 8) != 0) {
+    std::fclose(f);
+    return fail(std::string(path) + " is not a restart file of this library");
+  }
+  if (h.nthm != c->nthm || h.nzm != c->nzm || h.nrm != c->nrm || h.nsst != c->p.nsst ||
+      h.istr != c->p.istr || h.rint != c->p.rint || h.rext != c->p.rext || h.alx3 != c->p.alx3 ||
+      h.lamb != c->p.lamb || h.str != c->p.str) {
+    std::fclose(f);
+    return fail(std::string(path) + " was written for another grid or geometry");
+  }
+  std::vector<double> buf((size_t)c->N);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q : kRestartFields) {
+    if (std::fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) {
+      std::fclose(f);
+      return fail(std::string(path) + " is truncated");
+    }
+    HIPCHK(hipMemcpyAsync(c->f[q], buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  std::fclose(f);
+  c->time = h.time;
+  c->dt = h.dt;
+  c->cfl_valid = h.cfl_valid != 0;
+  c->cfl_cached = h.cfl;
+  if (ntime) *ntime = h.ntime;
   return 0;
 }
 

@@ -8,12 +8,15 @@
 ! CalcVelocityStats); every routine forwards to the C-ABI of libtcamd.so
 ! (include/tc_abi.h), where the state lives on the device.
 !
-! Usage: tc_host [--fused] [--ic perturbed SEED AMP]
+! Usage: tc_host [--fused] [--ic perturbed SEED AMP] [--restart FILE]
+!                [--checkpoint FILE]
 !   --fused      one fused device call per time step (tc_time_marcher)
 !                instead of the routine-by-routine calls
 !   --ic ...     seeded perturbed Couette start, projected divergence free;
 !                without it NREAD=0 uses CreateInitialConditions and
 !                NREAD=1 reads continua_{qt,qr,qz}.bin
+!   --restart F  continue from a tc_write_restart checkpoint (bit for bit)
+!   --checkpoint F  write a checkpoint of the final state
 ! Outputs: torquenu.out (time, inner, outer; e14.6 as CalcTorque.F90:59),
 !          torquenu_full.out (same at full precision), vmax.out,
 !          continua_{qt,qr,qz,pr}.bin at exit (raw FP64, (nr,nzm,nthm)).
@@ -170,6 +173,23 @@ module tc_abi
       import
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function tc_get_scalars(ctx, out8) bind(C, name='tc_get_scalars')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out8(8)
+    end function
+    integer(c_int) function tc_write_restart(ctx, path, ntime) bind(C, name='tc_write_restart')
+      import
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: ntime
+    end function
+    integer(c_int) function tc_read_restart(ctx, path, ntime) bind(C, name='tc_read_restart')
+      import
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: ntime
+    end function
   end interface
 contains
   subroutine tc_check(ierr, where)
@@ -204,6 +224,8 @@ module param
   integer :: ntime
   logical :: variabletstep = .true., readflow = .false., use_fused = .false.
   logical :: perturbed_ic = .false.
+  character(len=512) :: restart_file = '', checkpoint_file = ''
+  integer :: ntime0 = 0
   integer(c_long_long) :: ic_seed = 1
   real :: ic_amp = 1.d-2
   real :: vmax(3)
@@ -419,6 +441,21 @@ subroutine CalcTorque
   write(96, '(3(1x,es24.16))') time, anusslow, anussup
 end subroutine
 
+! Whole-state checkpoint of the library (fields, pressure, RK histories,
+! time, step index): the run continues bit for bit from step ntime0
+subroutine RestartFromCheckpoint
+  use param
+  use tc_abi
+  implicit none
+  integer(c_int) :: n
+  real(c_double) :: s(8)
+  call tc_check(tc_read_restart(ctx, trim(restart_file)//c_null_char, n), 'restart')
+  call tc_check(tc_get_scalars(ctx, s), 'tc_get_scalars')
+  ntime0 = n
+  time = s(1)
+  write(6, '(a,a,a,i0,a,es12.5)') 'restart from ', trim(restart_file), ' at step ', n, ' time ', time
+end subroutine
+
 ! TimeMarcher.F90:12-57; the halo updates have no device counterpart
 subroutine TimeMarcher
   use param
@@ -463,6 +500,12 @@ program tc_host
     call get_command_argument(a, arg)
     if (trim(arg) == '--fused') then
       use_fused = .true.
+    else if (trim(arg) == '--restart') then
+      call get_command_argument(a + 1, restart_file)
+      a = a + 1
+    else if (trim(arg) == '--checkpoint') then
+      call get_command_argument(a + 1, checkpoint_file)
+      a = a + 1
     else if (trim(arg) == '--ic') then
       call get_command_argument(a + 1, arg)
       if (trim(arg) /= 'perturbed') error stop 'usage: --ic perturbed SEED AMP'
@@ -484,19 +527,21 @@ program tc_host
   open(96, file='torquenu_full.out', status='replace')
   open(95, file='vmax.out', status='replace')
 
-  if (perturbed_ic) then
+  time = 0.d0
+  if (len_trim(restart_file) > 0) then
+    call RestartFromCheckpoint
+  else if (perturbed_ic) then
     call PerturbedCouetteStart
   else if (readflow) then
     call ReadFlowField
   else
     call CreateInitialConditions
   end if
-  time = 0.d0
   call CheckDivergence(dmax)
   write(6, *) ' initial divg dmax  ', dmax
 
   errorcode = 0
-  do ntime = 0, ntst
+  do ntime = ntime0, ntst
     call CalcMaxCFL(instcfl)
     if (variabletstep) then
       if (ntime > 1) then
@@ -542,6 +587,12 @@ program tc_host
   write(6, '(a,i0,a,i0,a,es12.5,a,f10.3,a)') 'exit code ', errorcode, ' after step ', ntime, &
       ' time ', time, ' wall ', dble(c1 - c0) / dble(crate), ' s'
   call WriteFlowField
+  if (len_trim(checkpoint_file) > 0) then
+    ! the device holds the start time of the last step; store the time reached
+    call tc_check(tc_set_time(ctx, time, dt), 'tc_set_time')
+    call tc_check(tc_write_restart(ctx, trim(checkpoint_file)//c_null_char, int(ntime + 1, c_int)), &
+                  'checkpoint')
+  end if
   close(95); close(96); close(97)
   call tc_check(tc_destroy(ctx), 'tc_destroy')
   if (errorcode /= 333 .and. errorcode /= 334 .and. errorcode /= 555 .and. errorcode /= 0) stop 3

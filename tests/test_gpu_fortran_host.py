@@ -99,3 +99,31 @@ def test_error_codes_match_oracle(tc, oracle):
                     break
             codes.append(seq)
         assert codes[0] == codes[1], (case, codes)
+
+
+def test_fortran_host_checkpoint_restart(tc, tmp_path):
+    """tc_host --checkpoint after 3 steps, then --restart for the last 3: the
+    torque of every step and the final fields equal the uninterrupted run."""
+    if not os.path.exists(HOST):
+        pytest.fail("tc_host is not built")
+    p = _params(tc)
+    full = tmp_path / "full"
+    full.mkdir()
+    tq_full, f_full = _run_host(full, p, fused=True)
+    part = tmp_path / "part"
+    part.mkdir()
+    write_bou(part / "bou.in", dict(p, ntst=2))
+    r = subprocess.run([HOST, "--ic", "perturbed", "77", "0.01", "--fused", "--checkpoint", "ck.bin"],
+                       cwd=part, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "exit code 555" in r.stdout, r.stdout + r.stderr
+    tq_a = np.loadtxt(part / "torquenu_full.out", ndmin=2)
+    write_bou(part / "bou.in", p)
+    r = subprocess.run([HOST, "--restart", "ck.bin", "--fused"], cwd=part, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0 and "exit code 555" in r.stdout, r.stdout + r.stderr
+    tq_b = np.loadtxt(part / "torquenu_full.out", ndmin=2)
+    assert np.array_equal(np.vstack([tq_a, tq_b]), tq_full)
+    shape = (p["nrm"] + 1, p["nzm"], p["nthm"])
+    for n in ("qt", "qr", "qz", "pr"):
+        got = np.fromfile(part / f"continua_{n}.bin").reshape(shape, order="F")
+        assert np.array_equal(got, f_full[n]), n

@@ -55,20 +55,20 @@ int launch_status() {
 // ---------------------------------------------------------------------------
 void write_table(double *T, int nt, int n, std::vector<double> &dl, std::vector<double> &d,
                  std::vector<double> &du, const std::vector<double> &sc) {
-  const int stride = nt / 6;
+  constexpr int W = tc::LU_ROW;
   std::vector<double> du2((size_t)std::max(n, 1), 0.0);
   std::vector<int> ipiv((size_t)n);
   tc::dgttrf(n, dl.data(), d.data(), du.data(), du2.data(), ipiv.data());
   std::fill(T, T + nt, 0.0);
   for (int q = 0; q < n - 1; q++) {
-    T[q] = dl[q];
-    T[stride + q] = du[q];
-    T[5 * stride + q] = (ipiv[q] != q) ? 1.0 : 0.0;
+    T[W * q] = dl[q];
+    T[W * q + 1] = du[q];
+    T[W * q + 5] = (ipiv[q] != q) ? 1.0 : 0.0;
   }
-  for (int q = 0; q < n - 2; q++) T[2 * stride + q] = du2[q];
+  for (int q = 0; q < n - 2; q++) T[W * q + 2] = du2[q];
   for (int q = 0; q < n; q++) {
-    T[3 * stride + q] = 1.0 / d[q];
-    T[4 * stride + q] = sc[q];
+    T[W * q + 3] = 1.0 / d[q];
+    T[W * q + 4] = sc[q];
   }
 }
 
@@ -433,8 +433,37 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
                                        &g.ac3j,  &g.apph,  &g.amph,  &g.acph,  &g.ak1, &g.ak3};
     const int nv = (int)(sizeof vs / sizeof vs[0]);
     size_t stride = (size_t)std::max(c->nr, std::max(c->nthm, c->nzm)) + 2;
-    std::vector<double> h(stride * nv, 0.0);
+    stride = (stride + 7) / 8 * 8;  // keeps the plane records 64-byte aligned
+    constexpr size_t HPW = sizeof(tc::HelmPlane) / sizeof(double);
+    std::vector<double> h(stride * nv + HPW * (size_t)(c->nr + 1), 0.0);
     for (int q = 0; q < nv; q++) std::copy(vs[q]->begin(), vs[q]->end(), h.begin() + q * stride);
+    tc::HelmPlane *hp = reinterpret_cast<tc::HelmPlane *>(h.data() + stride * nv);
+    for (int k = 1; k <= c->nrm; k++) {
+      tc::HelmPlane &e = hp[k];
+      e.udxt_m = g.dxt * 0.25 * g.usrm[k];
+      e.udxtq_m = g.dxtq / (p->ren * (g.rm[k] * g.rm[k]));
+      e.udx2rm = g.udx2rm[k];
+      e.usrm = g.usrm[k];
+      e.usrc = g.usrc[k];
+      e.usrc_p = g.usrc[k + 1];
+      e.udx1vis1 = g.udx1vis1[k];
+      e.udxt_c = g.dxt * 0.25 * g.usrc[k];
+      e.udxtq_c = g.dxtq / (p->ren * (g.rc[k] * g.rc[k]));
+      e.usrc_m = g.usrc[k - 1];
+      e.udx2c = g.udx2c[k];
+      e.udx1vis2 = g.udx1vis2[k];
+      e.rc = g.rc[k];
+      e.ap1j = g.ap1j[k];
+      e.ac1j = g.ac1j[k];
+      e.am1j = g.am1j[k];
+      e.ap2je = g.ap2je[k];
+      e.ac2je = g.ac2je[k];
+      e.am2je = g.am2je[k];
+      e.ap3j = g.ap3j[k];
+      e.ac3j = g.ac3j[k];
+      e.am3j = g.am3j[k];
+      e.udxzq = g.dxzq / p->ren;
+    }
     TRY(dev_alloc(c, (void **)&c->gvbuf, sizeof(double) * h.size()));
     HTRY(hipMemcpyAsync(c->gvbuf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                         c->stream));
@@ -446,6 +475,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
                             &c->gd.ap3j, &c->gd.am3j, &c->gd.ac3j, &c->gd.apph, &c->gd.amph,
                             &c->gd.acph, &c->gd.ak1, &c->gd.ak3};
     for (int q = 0; q < nv; q++) *dst[q] = c->gvbuf + q * stride;
+    c->gd.hp = reinterpret_cast<const tc::HelmPlane *>(c->gvbuf + stride * nv);
     c->gd.dxt = g.dxt;
     c->gd.dxz = g.dxz;
     c->gd.dxtq = g.dxtq;
@@ -473,7 +503,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   tc::slip_bcs(c->p, c->g, 0.0, c->icyl, c->ocyl);
   TRY(upload_masks(c));
   // LU tables
-  c->lu_nt = 6 * c->nr;
+  c->lu_nt = tc::LU_ROW * c->nr;
   {
     const size_t n = (size_t)3 * 3 * 4 * c->lu_nt;
     TRY(dev_alloc(c, (void **)&c->lu, sizeof(double) * n));

@@ -14,7 +14,26 @@ namespace tc {
 
 // Device view of the grid: pointers into one device buffer of 1-based
 // vectors (v[k] == reference v(k)) plus the scalars the kernels need.
+// The radial coefficients of the explicit terms and right-hand sides at one
+// radial index k, gathered into one 192-byte record so a kernel marching
+// along r fetches them with three scalar loads per plane from one pointer.
+// Each entry is evaluated on the host with the expression (and operand
+// order) the kernels used to evaluate on the device.
+struct HelmPlane {
+  double udxt_m;   // dxt*0.25*usrm(k)
+  double udxtq_m;  // dxtq/(ren*(rm(k)*rm(k)))
+  double udx2rm, usrm, usrc, usrc_p, udx1vis1;  // usrc_p = usrc(k+1)
+  double udxt_c;   // dxt*0.25*usrc(k)
+  double udxtq_c;  // dxtq/(ren*(rc(k)*rc(k)))
+  double usrc_m, udx2c, udx1vis2, rc;           // usrc_m = usrc(k-1)
+  double ap1j, ac1j, am1j, ap2je, ac2je, am2je, ap3j, ac3j, am3j;
+  double udxzq;    // dxzq/ren
+  double pad;
+};
+static_assert(sizeof(HelmPlane) == 192, "HelmPlane is three 64-byte loads");
+
 struct GridDev {
+  const HelmPlane *hp;  // [k], k = 1..nrm
   const double *rc, *rm, *usrc, *usrm, *udx2rm, *udx2m, *udx2c, *rudx2c, *udx1vis1,
       *udx1vis2, *udx1qm, *pvol;
   const double *ap1j, *am1j, *ac1j, *ap2je, *am2je, *ac2je, *ap3j, *am3j, *ac3j;
@@ -37,8 +56,10 @@ struct FFTPlan {
 };
 
 // Helmholtz LU tables: per (substep slot, component, mask variant) a block of
-// 6*NT doubles: dl | du | du2 | rd=1/d | row scale | pivot flag.
+// NT rows of LU_ROW doubles: dl, du, du2, rd=1/d, row scale, pivot flag, 0, 0.
+// One row is one 64-byte scalar load.
 enum { LU_QT = 0, LU_QR = 1, LU_QZ = 2 };
+constexpr int LU_ROW = 8;
 
 struct Timer {
   std::string name;
@@ -110,7 +131,7 @@ struct tc_ctx {
   double *lu = nullptr;        // device
   double *lu_host = nullptr;   // pinned staging
   hipEvent_t lu_copied = nullptr;
-  int lu_nt = 0;               // doubles per table = 6*(nrm+1)
+  int lu_nt = 0;               // doubles per table = LU_ROW*(nrm+1)
   double lu_key[3] = {-1, -1, -1};
   bool lu_valid = false;
   // FFT

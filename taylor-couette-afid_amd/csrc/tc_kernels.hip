@@ -51,11 +51,12 @@ DEVI GAcc gacc(const GridDev &g, const double *a, int k, int j, int i) {
 
 // ExplicitTermsQT.F90:44-125 at (kc, j, i)
 template <class A>
-DEVI double explicit_qt_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
+DEVI double explicit_qt_g(const GridDev &g, const HelmPlane &c, const A &T, const A &R,
+                          const A &Z, int kc) {
   const int dkm = kc > 1 ? -1 : 0, dkp = kc < g.nrm ? 1 : 0;  // kmv, kpv clamps
-  const double udxt = g.dxt * 0.25 * g.usrm[kc];
-  const double udxtq = g.dxtq / (g.ren * (g.rm[kc] * g.rm[kc]));
-  const double udxzq = g.dxzq / g.ren;
+  const double udxt = c.udxt_m;
+  const double udxtq = c.udxtq_m;
+  const double udxzq = c.udxzq;
   const double udxz = g.dxz * 0.25;
   const double t_c = T(0, 0, 0), t_ip = T(0, 0, 1), t_im = T(0, 0, -1);
   const double t_jp = T(0, 1, 0), t_jm = T(0, -1, 0);
@@ -64,28 +65,29 @@ DEVI double explicit_qt_g(const GridDev &g, const A &T, const A &R, const A &Z, 
   const double r_kp = R(1, 0, 0), r_kpim = R(1, 0, -1);
   const double z_c = Z(0, 0, 0), z_im = Z(0, 0, -1);
   const double z_jp = Z(0, 1, 0), z_jpim = Z(0, 1, -1);
-  const double usrc_kp = g.usrc[kc + 1], usrc_k = g.usrc[kc];
+  const double usrc_kp = c.usrc_p, usrc_k = c.usrc;
 
   const double h11 = ((t_ip + t_c) * (t_ip + t_c) - (t_c + t_im) * (t_c + t_im)) * udxt;
-  const double h12d = ((r_kp + r_kpim) * (t_kp + t_c) - (r_c + r_im) * (t_c + t_km)) * 0.25 * g.udx2rm[kc];
-  const double h12n = t_c * g.usrm[kc] * ((r_kp + r_kpim) * usrc_kp + (r_c + r_im) * usrc_k) * 0.25;
+  const double h12d = ((r_kp + r_kpim) * (t_kp + t_c) - (r_c + r_im) * (t_c + t_km)) * 0.25 * c.udx2rm;
+  const double h12n = t_c * c.usrm * ((r_kp + r_kpim) * usrc_kp + (r_c + r_im) * usrc_k) * 0.25;
   const double h12 = h12d + h12n;
   const double h13 = ((z_jp + z_jpim) * (t_jp + t_c) - (z_c + z_im) * (t_c + t_jm)) * udxz;
   const double d11qt = (t_ip - 2.0 * t_c + t_im) * udxtq;
   const double d22qt = (t_jp - 2.0 * t_c + t_jm) * udxzq;
   const double qre = r_kp * usrc_kp + r_c * usrc_k;
   const double qrw = r_kpim * usrc_kp + r_im * usrc_k;
-  const double d11qre = (qre - qrw) * g.udx1vis1[kc] * 0.5;
+  const double d11qre = (qre - qrw) * c.udx1vis1 * 0.5;
   const double coriol = -((r_c + r_im) * usrc_k + (r_kp + r_kpim) * usrc_kp) * 0.25 * g.romeg;
   return -(h11 + h12 + h13) + d11qre + d22qt + d11qt + coriol;
 }
 
 // ExplicitTermsQR.F90:46-146 at (kc, j, i), kc = 2..nrm
 template <class A>
-DEVI double explicit_qr_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
-  const double udxt = g.dxt * 0.25 * g.usrc[kc];
-  const double udxtq = g.dxtq / (g.ren * (g.rc[kc] * g.rc[kc]));
-  const double udxzq = g.dxzq / g.ren;
+DEVI double explicit_qr_g(const GridDev &g, const HelmPlane &c, const A &T, const A &R,
+                          const A &Z, int kc) {
+  const double udxt = c.udxt_c;
+  const double udxtq = c.udxtq_c;
+  const double udxzq = c.udxzq;
   const double udxz = g.dxz * 0.25;
   const double t_c = T(0, 0, 0), t_ip = T(0, 0, 1);
   const double t_m = T(-1, 0, 0), t_mip = T(-1, 0, 1);
@@ -98,26 +100,27 @@ DEVI double explicit_qr_g(const GridDev &g, const A &T, const A &R, const A &Z, 
   const double tavg = (t_ip + t_mip + t_c + t_m) * 0.25;
   const double h11n = tavg * tavg;
   const double h21 = ((t_ip + t_mip) * (r_ip + r_c) - (t_c + t_m) * (r_c + r_im)) * udxt;
-  const double h22 = ((r_kp * g.usrc[kc + 1] + r_c * g.usrc[kc]) * (r_kp + r_c) -
-                      (r_c * g.usrc[kc] + r_km * g.usrc[kc - 1]) * (r_c + r_km)) *
-                     g.udx2c[kc] * 0.25;
+  const double h22 = ((r_kp * c.usrc_p + r_c * c.usrc) * (r_kp + r_c) -
+                      (r_c * c.usrc + r_km * c.usrc_m) * (r_c + r_km)) *
+                     c.udx2c * 0.25;
   const double h23 = ((z_jp + z_mjp) * (r_jp + r_c) - (z_c + z_m) * (r_c + r_jm)) * udxz;
   const double qte = (t_ip + t_mip);
   const double qtw = (t_c + t_m);
-  const double d11qre = -(qte - qtw) * g.udx1vis2[kc];
+  const double d11qre = -(qte - qtw) * c.udx1vis2;
   const double d11qr = (r_ip - 2.0 * r_c + r_im) * udxtq;
   const double d22qr = (r_jp - 2.0 * r_c + r_jm) * udxzq;
-  const double coriol = (t_c + t_ip + t_m + t_mip) * 0.25 * g.rc[kc] * g.romeg;
+  const double coriol = (t_c + t_ip + t_m + t_mip) * 0.25 * c.rc * g.romeg;
   return -(h21 + h22 + h23) + h11n + d11qr + d22qr + d11qre + coriol;
 }
 
 // ExplicitTermsQZ.F90:40-102 at (kc, j, i)
 template <class A>
-DEVI double explicit_qz_g(const GridDev &g, const A &T, const A &R, const A &Z, int kc) {
+DEVI double explicit_qz_g(const GridDev &g, const HelmPlane &c, const A &T, const A &R,
+                          const A &Z, int kc) {
   const int dkm = kc > 1 ? -1 : 0, dkp = kc < g.nrm ? 1 : 0;
-  const double udxt = g.dxt * 0.25 * g.usrm[kc];
-  const double udxtq = g.dxtq / (g.ren * (g.rm[kc] * g.rm[kc]));
-  const double udxzq = g.dxzq / g.ren;
+  const double udxt = c.udxt_m;
+  const double udxtq = c.udxtq_m;
+  const double udxzq = c.udxzq;
   const double udxz = g.dxz * 0.25;
   const double t_c = T(0, 0, 0), t_ip = T(0, 0, 1);
   const double t_jm = T(0, -1, 0), t_jmip = T(0, -1, 1);
@@ -128,7 +131,7 @@ DEVI double explicit_qz_g(const GridDev &g, const A &T, const A &R, const A &Z, 
   const double z_kp = Z(dkp, 0, 0), z_km = Z(dkm, 0, 0);
 
   const double h31 = ((t_ip + t_jmip) * (z_ip + z_c) - (t_c + t_jm) * (z_c + z_im)) * udxt;
-  const double h32 = (((r_kp + r_kpjm) * (z_kp + z_c)) - ((r_c + r_jm) * (z_c + z_km))) * g.udx2rm[kc] * 0.25;
+  const double h32 = (((r_kp + r_kpjm) * (z_kp + z_c)) - ((r_c + r_jm) * (z_c + z_km))) * c.udx2rm * 0.25;
   const double h33 = ((z_jp + z_c) * (z_jp + z_c) - (z_c + z_jm) * (z_c + z_jm)) * udxz;
   const double dqz1 = (z_im - 2.0 * z_c + z_ip) * udxtq;
   const double dqz2 = (z_jm - 2.0 * z_c + z_jp) * udxzq;
@@ -145,13 +148,14 @@ __global__ __launch_bounds__(256) void k_explicit(GridDev g, const double *__res
   if (i >= g.nthm || j >= g.nzm) return;
   const int kc = blockIdx.z + (WHICH == 1 ? 2 : 1);
   const GAcc T = gacc(g, qt, kc, j, i), R = gacc(g, qr, kc, j, i), Z = gacc(g, qz, kc, j, i);
+  const HelmPlane c = g.hp[kc];
   double v;
   if (WHICH == 0)
-    v = explicit_qt_g(g, T, R, Z, kc);
+    v = explicit_qt_g(g, c, T, R, Z, kc);
   else if (WHICH == 1)
-    v = explicit_qr_g(g, T, R, Z, kc);
+    v = explicit_qr_g(g, c, T, R, Z, kc);
   else
-    v = explicit_qz_g(g, T, R, Z, kc);
+    v = explicit_qz_g(g, c, T, R, Z, kc);
   out[IX(g, kc, j, i)] = v;
 }
 
@@ -189,57 +193,62 @@ void launch_explicit_qz(tc_ctx *c) {
 // ---------------------------------------------------------------------------
 
 struct LUView {
-  const double *dl, *du, *du2, *rd, *sc, *pv;
+  const double *b;  // rows of LU_ROW doubles: dl du du2 rd sc pv
+  DEVI double dl(int r) const { return b[LU_ROW * r]; }
+  DEVI double du(int r) const { return b[LU_ROW * r + 1]; }
+  DEVI double du2(int r) const { return b[LU_ROW * r + 2]; }
+  DEVI double rd(int r) const { return b[LU_ROW * r + 3]; }
+  DEVI double sc(int r) const { return b[LU_ROW * r + 4]; }
+  DEVI double pv(int r) const { return b[LU_ROW * r + 5]; }
 };
 DEVI LUView lu_view(const double *base, int nt) {
-  const int n = nt / 6;
   LUView v;
-  v.dl = base;
-  v.du = base + n;
-  v.du2 = base + 2 * n;
-  v.rd = base + 3 * n;
-  v.sc = base + 4 * n;
-  v.pv = base + 5 * n;
+  v.b = base;
   return v;
 }
 
 // streamed L-solve step for row r (0-based): f is the right-hand side of row r+1
 DEVI void lsolve_step(const LUView &T, int r, double f, double &carry, double &out) {
-  if (T.pv[r] == 0.0) {
+  if (T.pv(r) == 0.0) {
     out = carry;
-    carry = f - T.dl[r] * carry;
+    carry = f - T.dl(r) * carry;
   } else {
     out = f;
-    carry = carry - T.dl[r] * f;
+    carry = carry - T.dl(r) * f;
   }
 }
 
 // Right-hand side row k of the q_theta solve (ImplicitAndUpdateQT.F90:45-115);
 // qm, q0, qp = qt(k-1), qt(k), qt(k+1) on the line.
-DEVI double rhs_qt(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
-                   double pr_c, double pr_im, double dqv, double ruold, double fic, double foc) {
+// NOSLIP: every wall mask is 1, so the blends with the free-slip
+// coefficients reduce to the no-slip ones (x*1 + y*0 == x for the nonzero
+// wall coefficients).
+template <bool NOSLIP = false>
+DEVI double rhs_qt(const GridDev &g, const HelmPlane &c, const ImpArgs &a, int k, double qm,
+                   double q0, double qp, double pr_c, double pr_im, double dqv, double ruold,
+                   double fic, double foc) {
   double d33;
   if (k == 1) {
-    const double app = g.ap1j[1];
-    const double acc = (g.ac1j[1] * fic + g.ac1j_fs1 * (1 - fic));
-    const double amm = (g.am1j[1] * fic + g.am1j_fs1 * (1 - fic));
+    const double app = c.ap1j;
+    const double acc = NOSLIP ? c.ac1j : (c.ac1j * fic + g.ac1j_fs1 * (1 - fic));
+    const double amm = NOSLIP ? c.am1j : (c.am1j * fic + g.am1j_fs1 * (1 - fic));
     d33 = qp * app + q0 * acc + a.vwall_in * amm;
   } else if (k < g.nrm) {
-    d33 = qp * g.ap1j[k] + q0 * g.ac1j[k] + qm * g.am1j[k];
+    d33 = qp * c.ap1j + q0 * c.ac1j + qm * c.am1j;
   } else {
-    const double app = (g.ap1j[k] * foc + g.ap1j_fs2 * (1 - foc));
-    const double acc = (g.ac1j[k] * foc + g.ac1j_fs2 * (1 - foc));
-    d33 = a.virotext * app + q0 * acc + qm * g.am1j[k];
+    const double app = NOSLIP ? c.ap1j : (c.ap1j * foc + g.ap1j_fs2 * (1 - foc));
+    const double acc = NOSLIP ? c.ac1j : (c.ac1j * foc + g.ac1j_fs2 * (1 - foc));
+    d33 = a.virotext * app + q0 * acc + qm * c.am1j;
   }
-  const double dpx = (pr_c - pr_im) * g.dxt * a.al * g.usrm[k];
+  const double dpx = (pr_c - pr_im) * g.dxt * a.al * c.usrm;
   return (a.ga * dqv + a.ro * ruold + a.alre * d33 - dpx) * a.dt;
 }
 
 // Right-hand side row k = 2..nrm of the q_r solve (ImplicitAndUpdateQR.F90:41-63).
-DEVI double rhs_qr(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
+DEVI double rhs_qr(const HelmPlane &c, const ImpArgs &a, double qm, double q0, double qp,
                    double pr_c, double pr_km, double dphv, double ruold) {
-  const double d33 = qp * g.ap2je[k] + q0 * g.ac2je[k] + qm * g.am2je[k];
-  const double udx2 = g.rc[k] * a.al * g.udx2c[k];
+  const double d33 = qp * c.ap2je + q0 * c.ac2je + qm * c.am2je;
+  const double udx2 = c.rc * a.al * c.udx2c;
   const double dpx = (pr_c - pr_km) * udx2;
   return (a.ga * dphv + a.ro * ruold + a.alre * d33 - dpx) * a.dt;
 }
@@ -248,20 +257,22 @@ DEVI double rhs_qr(const GridDev &g, const ImpArgs &a, int k, double qm, double 
 // Row nrm is the one the reference leaves behind: assembled a second time
 // after the loop overwrote ru3(nrm) with qcap(nrm), so its history term is
 // ro*qcap(nrm), with the loop's amm = am3j(nrm).
-DEVI double rhs_qz(const GridDev &g, const ImpArgs &a, int k, double qm, double q0, double qp,
-                   double pr_c, double pr_jm, double qcv, double ruold, double fic, double foc) {
+template <bool NOSLIP = false>
+DEVI double rhs_qz(const GridDev &g, const HelmPlane &c, const ImpArgs &a, int k, double qm,
+                   double q0, double qp, double pr_c, double pr_jm, double qcv, double ruold,
+                   double fic, double foc) {
   double d33, hist = ruold;
   if (k == 1) {
-    const double app = g.ap3j[1];
-    const double acc = (g.ac3j[1] * fic + g.ac3j_fs1 * (1 - fic));
-    const double amm = (g.am3j[1] * fic + g.am3j_fs1 * (1 - fic));
+    const double app = c.ap3j;
+    const double acc = NOSLIP ? c.ac3j : (c.ac3j * fic + g.ac3j_fs1 * (1 - fic));
+    const double amm = NOSLIP ? c.am3j : (c.am3j * fic + g.am3j_fs1 * (1 - fic));
     d33 = qp * app + q0 * acc + q0 * amm;
   } else if (k < g.nrm) {
-    d33 = qp * g.ap3j[k] + q0 * g.ac3j[k] + qm * g.am3j[k];
+    d33 = qp * c.ap3j + q0 * c.ac3j + qm * c.am3j;
   } else {
-    const double app = (g.ap3j[k] * foc + g.ap3j_fs2 * (1 - foc));
-    const double acc = (g.ac3j[k] * foc + g.ac3j_fs2 * (1 - foc));
-    d33 = q0 * app + q0 * acc + qm * g.am3j[k];
+    const double app = NOSLIP ? c.ap3j : (c.ap3j * foc + g.ap3j_fs2 * (1 - foc));
+    const double acc = NOSLIP ? c.ac3j : (c.ac3j * foc + g.ac3j_fs2 * (1 - foc));
+    d33 = q0 * app + q0 * acc + qm * c.am3j;
     hist = qcv;
   }
   const double dpx = (pr_c - pr_jm) * a.udxz;
@@ -296,12 +307,12 @@ __global__ __launch_bounds__(256) void k_implicit_qt(GridDev g, ImpArgs a, doubl
       qp = (k < n) ? qt[o + P] : 0.0;
     }
     const double dqv = dq[o];
-    const double r = rhs_qt(g, a, k, qm, q0, qp, pr[o], pr[o - l + lim], dqv, ru1[o], fic, foc);
+    const double r = rhs_qt(g, g.hp[k], a, k, qm, q0, qp, pr[o], pr[o - l + lim], dqv, ru1[o], fic, foc);
     ru1[o] = dqv;
     if (k == 1) {
-      carry = (r + a.bc1) * T.sc[0];
+      carry = (r + a.bc1) * T.sc(0);
     } else {
-      lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
+      lsolve_step(T, k - 2, r * T.sc(k - 1), carry, yprev);
       y[o - P] = yprev;
     }
   }
@@ -330,9 +341,9 @@ __global__ __launch_bounds__(256) void k_implicit_qr(GridDev g, ImpArgs a,
     q0 = qp;
     qp = qr[o + P];
     const double dv = dph[o];
-    const double r = rhs_qr(g, a, k, qm, q0, qp, pr[o], pr[o - P], dv, ru2[o]);
+    const double r = rhs_qr(g.hp[k], a, qm, q0, qp, pr[o], pr[o - P], dv, ru2[o]);
     ru2[o] = dv;
-    lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
+    lsolve_step(T, k - 2, r * T.sc(k - 1), carry, yprev);
     y[o - P] = yprev;
   }
   lsolve_step(T, n - 2, 0.0, carry, yprev);  // row nr: zero right-hand side
@@ -369,12 +380,12 @@ __global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a,
       qp = (k < n) ? qz[o + P] : 0.0;
     }
     const double qv = qcap[o];
-    const double r = rhs_qz(g, a, k, qm, q0, qp, pr[o], pr[o - l + ljm], qv, ru3[o], fic, foc);
+    const double r = rhs_qz(g, g.hp[k], a, k, qm, q0, qp, pr[o], pr[o - l + ljm], qv, ru3[o], fic, foc);
     ru3[o] = qv;
     if (k == 1) {
-      carry = r * T.sc[0];
+      carry = r * T.sc(0);
     } else {
-      lsolve_step(T, k - 2, r * T.sc[k - 1], carry, yprev);
+      lsolve_step(T, k - 2, r * T.sc(k - 1), carry, yprev);
       y[o - P] = yprev;
     }
   }
@@ -387,9 +398,9 @@ __global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a,
 DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
                       const double *__restrict__ y, long long l, long long P) {
   constexpr int CH = 8;
-  double x1 = y[(long long)(n - 1) * P + l] * T.rd[n - 1];
+  double x1 = y[(long long)(n - 1) * P + l] * T.rd(n - 1);
   if (n - 1 >= first) q[(long long)(n - 1) * P + l] += x1;
-  double x0 = (y[(long long)(n - 2) * P + l] - T.du[n - 2] * x1) * T.rd[n - 2];
+  double x0 = (y[(long long)(n - 2) * P + l] - T.du(n - 2) * x1) * T.rd(n - 2);
   if (n - 2 >= first) q[(long long)(n - 2) * P + l] += x0;
   for (int r0 = n - 3; r0 >= first; r0 -= CH) {
     double yv[CH], qv[CH];
@@ -406,7 +417,7 @@ DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
     for (int u = 0; u < CH; u++) {
       const int r = r0 - u;
       if (r >= first) {
-        const double x = (yv[u] - T.du[r] * x0 - T.du2[r] * x1) * T.rd[r];
+        const double x = (yv[u] - T.du(r) * x0 - T.du2(r) * x1) * T.rd(r);
         q[(long long)r * P + l] = qv[u] + x;
         x1 = x0;
         x0 = x;
@@ -507,14 +518,15 @@ DEVI void helm_fwd_body(
   const int i0 = (tile % ntx) * FT_I, j0 = (tile / ntx) * TJ;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const long long P = g.P;
-  // the tile elements this thread stages per plane
+  // the tile elements this thread stages per plane: chunks 0..NE-2 are
+  // full; in the last one threads past S load a valid element and do not
+  // store it, so every global load is unconditional
   long long off[NE];
-  bool has[NE];
+  const bool has_last = threadIdx.x + (NE - 1) * 256 < S;
 #pragma unroll
   for (int q = 0; q < NE; q++) {
     const int e = threadIdx.x + q * 256;
-    has[q] = e < S;
-    const int ee = has[q] ? e : 0;
+    const int ee = e < S ? e : 0;
     int jj = j0 + ee / FT_W - 1, ii = i0 + ee % FT_W - 1;
     jj = ((jj % g.nzm) + g.nzm) % g.nzm;
     ii = ((ii % g.nthm) + g.nthm) % g.nthm;
@@ -549,15 +561,19 @@ DEVI void helm_fwd_body(
   // planes 1 and 2 of the velocity, plane 1 of the pressure
 #pragma unroll
   for (int q = 0; q < NE; q++) {
-    if (!has[q]) continue;
-    const int e = threadIdx.x + q * 256;
-    sq[0][1][e] = qt[off[q]];
-    sq[1][1][e] = qr[off[q]];
-    sq[2][1][e] = qz[off[q]];
-    sq[0][2][e] = qt[P + off[q]];
-    sq[1][2][e] = qr[P + off[q]];
-    sq[2][2][e] = qz[P + off[q]];
-    sp[1][e] = pr[off[q]];
+    const double v0 = qt[off[q]], v1 = qr[off[q]], v2 = qz[off[q]];
+    const double w0 = qt[P + off[q]], w1 = qr[P + off[q]], w2 = qz[P + off[q]];
+    const double v3 = pr[off[q]];
+    if (q < NE - 1 || has_last) {
+      const int e = threadIdx.x + q * 256;
+      sq[0][1][e] = v0;
+      sq[1][1][e] = v1;
+      sq[2][1][e] = v2;
+      sq[0][2][e] = w0;
+      sq[1][2][e] = w1;
+      sq[2][2][e] = w2;
+      sp[1][e] = v3;
+    }
   }
   __syncthreads();
   const int nrm = g.nrm;
@@ -565,31 +581,27 @@ DEVI void helm_fwd_body(
 
   for (int k = 1; k <= nrm; k++) {
     // prefetch plane k+2 of the velocity, plane k+1 of the pressure and of
-    // the RK histories
+    // the RK histories.  In the last iteration these re-read the last plane
+    // (their values are not used), so the loads carry no condition.
     double nq[3][NE], np[NE], n1[LPT], n2[LPT], n3[LPT];
-    const bool pf = (k + 2 <= g.nr), pfp = (k + 1 <= nrm);
+    const long long oq = (long long)(k + 2 <= g.nr ? k + 1 : g.nr - 1) * P;
+    const long long op = (long long)(k + 1 <= nrm ? k : nrm - 1) * P;
 #pragma unroll
     for (int q = 0; q < NE; q++) {
-      if (has[q] && pf) {
-        const long long o = (long long)(k + 1) * P + off[q];
-        nq[0][q] = qt[o];
-        nq[1][q] = qr[o];
-        nq[2][q] = qz[o];
-      }
-      if (has[q] && pfp) np[q] = pr[(long long)k * P + off[q]];
+      nq[0][q] = qt[oq + off[q]];
+      nq[1][q] = qr[oq + off[q]];
+      nq[2][q] = qz[oq + off[q]];
+      np[q] = pr[op + off[q]];
     }
 #pragma unroll
     for (int u = 0; u < LPT; u++) {
-      n1[u] = n2[u] = n3[u] = 0.0;
-      if (valid[u] && pfp) {
-        const long long on = (long long)k * P + l[u];
-        n1[u] = ru1[on];
-        n2[u] = ru2[on];
-        n3[u] = ru3[on];
-      }
+      n1[u] = ru1[op + l[u]];
+      n2[u] = ru2[op + l[u]];
+      n3[u] = ru3[op + l[u]];
     }
     const int sm = (k + 2) % 3, sc = k % 3, sn = (k + 1) % 3;  // planes k-1, k, k+1
     const double *spk = sp[k & 1];
+    const HelmPlane hc = g.hp[k];
 #pragma unroll
     for (int u = 0; u < LPT; u++) {
       if (!valid[u]) continue;
@@ -601,35 +613,35 @@ DEVI void helm_fwd_body(
       const long long o = (long long)(k - 1) * P + l[u];
       // q_theta
       {
-        const double dqv = explicit_qt_g(g, T, R, Z, k);
+        const double dqv = explicit_qt_g(g, hc, T, R, Z, k);
         const double qm = (k > 1) ? T(-1, 0, 0) : 0.0, qp = (k < nrm) ? T(1, 0, 0) : 0.0;
-        const double r = rhs_qt(g, a, k, qm, T(0, 0, 0), qp, pr_c, pr_im, dqv, h1[u], fic[u], foc[u]);
+        const double r = rhs_qt<UNIFORM>(g, hc, a, k, qm, T(0, 0, 0), qp, pr_c, pr_im, dqv, h1[u], fic[u], foc[u]);
         ru1[o] = dqv;
         if (k == 1) {
-          ct[u] = (r + a.bc1) * Tt[u].sc[0];
+          ct[u] = (r + a.bc1) * Tt[u].sc(0);
         } else {
-          lsolve_step(Tt[u], k - 2, r * Tt[u].sc[k - 1], ct[u], yv);
+          lsolve_step(Tt[u], k - 2, r * Tt[u].sc(k - 1), ct[u], yv);
           yt[o - P] = yv;
         }
       }
       // q_r (rows 2..nrm; row 1 is the wall with zero right-hand side)
       if (k >= 2) {
-        const double dphv = explicit_qr_g(g, T, R, Z, k);
-        const double r = rhs_qr(g, a, k, R(-1, 0, 0), R(0, 0, 0), R(1, 0, 0), pr_c, pr_km[u], dphv, h2[u]);
+        const double dphv = explicit_qr_g(g, hc, T, R, Z, k);
+        const double r = rhs_qr(hc, a, R(-1, 0, 0), R(0, 0, 0), R(1, 0, 0), pr_c, pr_km[u], dphv, h2[u]);
         ru2[o] = dphv;
-        lsolve_step(Tr, k - 2, r * Tr.sc[k - 1], cr[u], yv);
+        lsolve_step(Tr, k - 2, r * Tr.sc(k - 1), cr[u], yv);
         yr[o - P] = yv;
       }
       // q_z
       {
-        const double qcv = explicit_qz_g(g, T, R, Z, k);
+        const double qcv = explicit_qz_g(g, hc, T, R, Z, k);
         const double qm = (k > 1) ? Z(-1, 0, 0) : 0.0, qp = (k < nrm) ? Z(1, 0, 0) : 0.0;
-        const double r = rhs_qz(g, a, k, qm, Z(0, 0, 0), qp, pr_c, pr_jm, qcv, h3[u], fic[u], foc[u]);
+        const double r = rhs_qz<UNIFORM>(g, hc, a, k, qm, Z(0, 0, 0), qp, pr_c, pr_jm, qcv, h3[u], fic[u], foc[u]);
         ru3[o] = qcv;
         if (k == 1) {
-          cz[u] = r * Tz[u].sc[0];
+          cz[u] = r * Tz[u].sc(0);
         } else {
-          lsolve_step(Tz[u], k - 2, r * Tz[u].sc[k - 1], cz[u], yv);
+          lsolve_step(Tz[u], k - 2, r * Tz[u].sc(k - 1), cz[u], yv);
           yz[o - P] = yv;
         }
       }
@@ -642,16 +654,17 @@ DEVI void helm_fwd_body(
       h3[u] = n3[u];
     }
     __syncthreads();
+    // in the last iteration these fill slots no later plane reads
+    const int s2 = (k + 2) % 3;
 #pragma unroll
     for (int q = 0; q < NE; q++) {
       const int e = threadIdx.x + q * 256;
-      if (has[q] && pf) {
-        const int s2 = (k + 2) % 3;
+      if (q < NE - 1 || has_last) {
         sq[0][s2][e] = nq[0][q];
         sq[1][s2][e] = nq[1][q];
         sq[2][s2][e] = nq[2][q];
+        sp[(k + 1) & 1][e] = np[q];
       }
-      if (has[q] && pfp) sp[(k + 1) & 1][e] = np[q];
     }
     __syncthreads();
   }

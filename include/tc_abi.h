@@ -208,6 +208,36 @@ int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
 int tc_write_restart(tc_ctx *ctx, const char *path, int ntime);
 int tc_read_restart(tc_ctx *ctx, const char *path, int *ntime);
 
+/* ---------------- statistics ---------------- */
+
+/* The stat_arrays accumulators of StatRoutines.F90 and CalcDissipation.F90,
+ * in WriteStats order (StatRoutines.F90:97-152; the file stems are the
+ * names); each is an (nrm, nzm) array in Fortran order (r fastest). */
+enum {
+  TC_ST_VT_MEAN = 0, TC_ST_VR_MEAN, TC_ST_VZ_MEAN, TC_ST_WT_MEAN, TC_ST_WR_MEAN,
+  TC_ST_WZ_MEAN, TC_ST_VT_RMS, TC_ST_VR_RMS, TC_ST_VZ_RMS, TC_ST_WT_RMS, TC_ST_WR_RMS,
+  TC_ST_WZ_RMS, TC_ST_VTVR_MEAN, TC_ST_VTVZ_MEAN, TC_ST_VRVZ_MEAN, TC_ST_PR_MEAN,
+  TC_ST_PR_RMS, TC_ST_DISSTE, TC_NSTATS
+};
+
+/* CalcStats (StatRoutines.F90:12-71) with the vorticity of CalcVorticity
+ * (CalcVorticity.F90:11-88): adds one sample of the theta-means to the
+ * accumulators and counts it (nstatsamples). */
+int tc_calc_stats(tc_ctx *ctx);
+/* CalcDissipation (CalcDissipation.F90:11-153): adds dissipation/Re/nthm
+ * to disste and returns nute, the volume-mean dissipation over the laminar
+ * Couette value (written to dissipnu.out by the reference, :150). */
+int tc_calc_dissipation(tc_ctx *ctx, double *nute);
+/* The accumulators, TC_NSTATS x nrm x nzm doubles, and the sample count
+ * (the data WriteStats reduces and writes, StatRoutines.F90:73-171). */
+int tc_get_stats(tc_ctx *ctx, double *out, int *nsamples);
+/* Replace the accumulators and the count: continuing statistics from
+ * earlier files (StatReadReduceWrite.F90:27-31 adds the old ones). */
+int tc_set_stats(tc_ctx *ctx, const double *in, int nsamples);
+/* nute of the last CalcDissipation (tc_calc_dissipation or the statistics
+ * of tc_step once time > TSTA, main.F90:188-190); NaN before the first. */
+int tc_last_dissipation(tc_ctx *ctx, double *nute);
+
 /* ---------------- batched sweep (no counterpart in the reference) ---------------- */
 
 /* Many independent instances, each a tc_ctx of its own shape and

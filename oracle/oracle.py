@@ -19,6 +19,10 @@ VECS = ["rc", "rm", "g2rc", "g2rm", "usrc", "usrm", "udx2rm", "udx2m", "udx2c",
         "rudx2c", "udx1vis1", "udx1vis2", "udx1qm", "ap1j", "am1j", "ac1j",
         "ap2j", "am2j", "ac2j", "ap2je", "am2je", "ac2je", "ap3j", "am3j",
         "ac3j", "apph", "amph", "acph", "ak1", "ak3", "fs", "pvol"]
+# stat_arrays in WriteStats order (StatRoutines.F90:97-152), file stems
+STATS = ["vt_mean", "vr_mean", "vz_mean", "wt_mean", "wr_mean", "wz_mean",
+         "vt_rms", "vr_rms", "vz_rms", "wt_rms", "wr_rms", "wz_rms",
+         "vtvr_mean", "vtvz_mean", "vrvz_mean", "pr_mean", "pr_rms", "disste"]
 
 PARAM_FIELDS = [
     ("nthm", ctypes.c_int), ("nzm", ctypes.c_int), ("nrm", ctypes.c_int),
@@ -87,6 +91,13 @@ def lib():
         L.or_check_divergence.argtypes = [ctypes.c_void_p]
         L.or_calc_torque.argtypes = [ctypes.c_void_p, _D, _D]
         L.or_calc_velocity_stats.argtypes = [ctypes.c_void_p, _D, _D]
+        L.or_calc_stats.argtypes = [ctypes.c_void_p]
+        L.or_calc_dissipation.restype = ctypes.c_double
+        L.or_calc_dissipation.argtypes = [ctypes.c_void_p]
+        L.or_get_stats.restype = ctypes.c_int
+        L.or_get_stats.argtypes = [ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int)]
+        L.or_last_nute.restype = ctypes.c_double
+        L.or_last_nute.argtypes = [ctypes.c_void_p]
         L.or_step.restype = ctypes.c_int
         L.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]
         for n in ["or_dgttrf", "or_zgttrf"]:
@@ -197,6 +208,26 @@ class Oracle:
         v, r = np.zeros(3), np.zeros(3)
         lib().or_calc_velocity_stats(self.h, _dp(v), _dp(r))
         return v, r
+
+    def calc_stats(self):
+        lib().or_calc_stats(self.h)
+
+    def calc_dissipation(self):
+        return lib().or_calc_dissipation(self.h)
+
+    def last_nute(self):
+        return lib().or_last_nute(self.h)
+
+    def stats(self):
+        """dict name -> (nrm, nzm) accumulator, and the sample count"""
+        out = np.zeros((len(STATS), self.nrm, self.nzm), order="C")
+        buf = np.zeros(len(STATS) * self.nrm * self.nzm)
+        n = ctypes.c_int()
+        lib().or_get_stats(self.h, _dp(buf), ctypes.byref(n))
+        for q, name in enumerate(STATS):
+            out[q] = buf[q * self.nrm * self.nzm:(q + 1) * self.nrm * self.nzm].reshape(
+                (self.nrm, self.nzm), order="F")
+        return {name: out[q] for q, name in enumerate(STATS)}, n.value
 
     def step(self, ntime, walltime=0.0):
         o = np.zeros(9)

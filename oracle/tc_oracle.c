@@ -13,7 +13,12 @@
  *   - CorrectPressure uses apph on both radial neighbours and be=beta
  *     (CorrectPressure.F90:23,40-46);
  *   - SolveImpEqnUpdate_QT scales row 1 with the unblended ac1j(1)
- *     (SolveImpEqnUpdate_QT.F90:65-67).
+ *     (SolveImpEqnUpdate_QT.F90:65-67);
+ *   - CalcStats pairs qt with qz(k,j)+qr(k,j+1) in vtvz_mean
+ *     (StatRoutines.F90:59-60);
+ *   - CalcDissipation's e12 takes (1/r) d q_r/d theta = d v_r/d theta, with
+ *     no 1/r (CalcDissipation.F90:78-80), and rm(nr), never assigned by
+ *     CreateGrid, is 0 in udx2mm(nrm).
  * Semantics are those of the non-OpenMP build (udxz = dxz*al and ocylsl as
  * set by SlipBCs in ImplicitAndUpdateQZ).
  *
@@ -63,6 +68,10 @@ struct or_state {
   int *icylsl, *ocylsl; /* (nzm, nthm), 1-based via MASK() */
   /* 3-D fields */
   double *f[OR_NFIELDS];
+  /* stat_arrays: OR_NSTATS accumulators of (nrm, nzm), Fortran order */
+  double *stats;
+  int nstatsamples;
+  double last_nute;
 };
 
 #define HIDX(s, k, j, i) \
@@ -450,6 +459,8 @@ or_state *or_create(const or_params *p) {
       return NULL;
     }
   }
+  s->stats = (double *)calloc((size_t)OR_NSTATS * s->nrm * s->nzm, sizeof(double));
+  s->last_nute = NAN;
   init_time_march_scheme(s);
   if (create_grid(s) != 0) {
     or_destroy(s);
@@ -472,6 +483,7 @@ void or_destroy(or_state *s) {
   free(s->kmc); free(s->kpc); free(s->kmv); free(s->kpv); free(s->kup); free(s->kum);
   free(s->icylsl); free(s->ocylsl);
   for (int id = 0; id < OR_NFIELDS; id++) free(s->f[id]);
+  free(s->stats);
   free(s);
 }
 
@@ -1665,6 +1677,115 @@ void or_calc_velocity_stats(or_state *s, double *vmax, double *rms) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Statistics (StatRoutines.F90:12-71) with the vorticity of          */
+/* CalcVorticity.F90:11-88 and the dissipation of CalcDissipation.F90 */
+/* ------------------------------------------------------------------ */
+#define STAT(q, k, j) s->stats[(size_t)(q) * s->nrm * s->nzm + (size_t)((k) - 1) + (size_t)s->nrm * ((j) - 1)]
+
+/* CalcVorticity.F90:30-81 at (kc, jc, ic); halos as update_halo leaves them */
+static void vorticity_at(or_state *s, int kc, int jc, int ic, double *vt, double *vr, double *vz) {
+  const int im = ic - 1, ip = ic + 1, jm = jc - 1, jp = jc + 1;
+  const int kmm = s->kmv[kc], kpp = s->kpv[kc], kp = kc + 1;
+  const double dxz = s->dxz, dxt = s->dxt;
+  const double *usrc = s->usrc, *usrm = s->usrm, *rm = s->rm, *udx2mm = s->udx2mm;
+  double durdz = ((QR(kp, jp, ic) * usrc[kp] + QR(kc, jp, ic) * usrc[kc]) -
+                  (QR(kp, jm, ic) * usrc[kp] + QR(kc, jm, ic) * usrc[kc])) * dxz * 0.25;
+  double duzdr = ((QZ(kpp, jc, ic) + QZ(kpp, jp, ic)) - (QZ(kmm, jc, ic) + QZ(kmm, jp, ic))) * 0.5 *
+                 udx2mm[kc];
+  *vt = durdz - duzdr;
+  double duzdt = ((QZ(kc, jc, ip) + QZ(kc, jp, ip)) - (QZ(kc, jc, im) + QZ(kc, jp, im))) * 0.25 * dxt *
+                 usrm[kc];
+  double dutdz = ((QT(kc, jp, ic) + QT(kc, jp, ip)) - (QT(kc, jm, ic) + QT(kc, jm, ip))) * 0.25 * dxz;
+  *vr = duzdt - dutdz;
+  double dutdr = ((QT(kpp, jc, ic) + QT(kpp, jc, ip)) * rm[kpp] -
+                  (QT(kmm, jc, ic) + QT(kmm, jc, ip)) * rm[kmm]) * 0.5 * udx2mm[kc] * usrm[kc];
+  double durdt = ((QR(kc, jc, ip) * usrc[kc] + QR(kp, jc, ip) * usrc[kp]) -
+                  (QR(kc, jc, im) * usrc[kc] + QR(kp, jc, im) * usrc[kp])) * dxt * 0.25;
+  *vz = dutdr - durdt;
+}
+
+/* StatRoutines.F90:12-71 (the vortices computed first by CalcVorticity
+ * are evaluated at the point: same values).  vtvz_mean keeps the
+ * reference's qr(k,j+1,i) in place of qz(k,j+1,i) (StatRoutines.F90:60). */
+void or_calc_stats(or_state *s) {
+  const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
+  const double *usrc = s->usrc;
+  s->nstatsamples = s->nstatsamples + 1;
+  const double usnthm = 1.0 / nthm;
+  for (int i = 1; i <= nthm; i++)
+    for (int j = 1; j <= nzm; j++)
+      for (int k = 1; k <= nrm; k++) {
+        const int kp = s->kpv[k];
+        double vt, vr, vz;
+        vorticity_at(s, k, j, i, &vt, &vr, &vz);
+        const double qt = QT(k, j, i), qr = QR(k, j, i), qz = QZ(k, j, i), pr = PR(k, j, i);
+        STAT(OR_ST_VT_ME, k, j) += qt * usnthm;
+        STAT(OR_ST_VR_ME, k, j) += qr * usnthm * usrc[k];
+        STAT(OR_ST_VZ_ME, k, j) += qz * usnthm;
+        STAT(OR_ST_WT_ME, k, j) += vt * usnthm;
+        STAT(OR_ST_WR_ME, k, j) += vr * usnthm;
+        STAT(OR_ST_WZ_ME, k, j) += vz * usnthm;
+        STAT(OR_ST_VT_RMS, k, j) += qt * qt * usnthm;
+        STAT(OR_ST_VR_RMS, k, j) += (qr * usrc[k]) * (qr * usrc[k]) * usnthm;
+        STAT(OR_ST_VZ_RMS, k, j) += qz * qz * usnthm;
+        STAT(OR_ST_WT_RMS, k, j) += vt * vt * usnthm;
+        STAT(OR_ST_WR_RMS, k, j) += vr * vr * usnthm;
+        STAT(OR_ST_WZ_RMS, k, j) += vz * vz * usnthm;
+        STAT(OR_ST_PR_ME, k, j) += pr * usnthm;
+        STAT(OR_ST_PR_RMS, k, j) += pr * pr * usnthm;
+        STAT(OR_ST_VTVR_ME, k, j) += (QT(k, j, i) + QT(k, j, i + 1)) *
+                                     (QR(k, j, i) * usrc[k] + QR(kp, j, i) * usrc[kp]) * usnthm * 0.25;
+        STAT(OR_ST_VTVZ_ME, k, j) += (QT(k, j, i) + QT(k, j, i + 1)) * (QZ(k, j, i) + QR(k, j + 1, i)) *
+                                     usnthm * 0.25;
+        STAT(OR_ST_VRVZ_ME, k, j) += (QZ(k, j, i) + QZ(k, j + 1, i)) *
+                                     (QR(k, j, i) * usrc[k] + QR(kp, j, i) * usrc[kp]) * usnthm * 0.25;
+      }
+}
+
+/* CalcDissipation.F90:37-150: accumulates disste and returns nute */
+double or_calc_dissipation(or_state *s) {
+  const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
+  const double dxt = s->dxt, dxz = s->dxz, pi = s->pi, ren = s->p.ren;
+  const double *usrc = s->usrc, *usrm = s->usrm, *rm = s->rm, *udx2mm = s->udx2mm,
+               *udx2cc = s->udx2cc;
+  double my_dissipte = 0.0;
+  for (int i = 1; i <= nthm; i++) {
+    const int imm = i - 1, ipp = i + 1;
+    for (int j = 1; j <= nzm; j++) {
+      const int jmm = j - 1, jpp = j + 1;
+      for (int k = 1; k <= nrm; k++) {
+        const int kp = k + 1, kpp = s->kpv[k], kmm = s->kmv[k];
+        double e11 = (QT(k, j, ipp) - QT(k, j, imm)) * dxt * usrm[k] * 0.5 + QR(k, j, i) * (usrc[k] * usrc[k]);
+        double e12 = (QT(kpp, j, i) * usrm[kpp] - QT(kmm, j, i) * usrm[kmm]) * udx2mm[k] * rm[k] +
+                     (QR(k, j, ipp) - QR(k, j, imm)) * usrc[k] * dxt * 0.5;
+        double e13 = (QT(k, jpp, i) - QT(k, jmm, i)) * 0.5 * dxz +
+                     (QZ(k, j, ipp) - QZ(k, j, imm)) * 0.5 * dxt * usrm[k];
+        double e22 = (QR(kp, j, i) * usrc[kp] - QR(kmm, j, i) * usrc[kmm]) * udx2cc[k];
+        double e23 = (QZ(kpp, j, i) - QZ(kmm, j, i)) * udx2mm[k] + (QR(k, jpp, i) - QR(k, jmm, i)) * usrc[k] * dxz * 0.5;
+        double e33 = (QZ(k, jpp, i) - QZ(k, jmm, i)) * dxz * 0.5;
+        double dissipte = 2.0 * (e11 * e11 + e22 * e22 + e33 * e33) + e12 * e12 + e13 * e13 + e23 * e23;
+        my_dissipte = my_dissipte + dissipte * s->pvol[k] * pi;
+        STAT(OR_ST_DISSTE, k, j) += dissipte / ren / (double)nthm;
+      }
+    }
+  }
+  const double rext = s->p.rext, rint = s->p.rint;
+  const double volt = pi * (rext * rext - rint * rint);
+  const double w = (s->p.virotint / rint - s->p.virotext / rext) / (rext - rint);
+  const double lamdissip = 4.0 * (rext * rext) * (rint * rint) / ((rext + rint) * (rext + rint)) * (w * w);
+  s->last_nute = my_dissipte / ((double)nthm * (double)nzm * volt * lamdissip);
+  return s->last_nute;
+}
+
+int or_get_stats(or_state *s, double *out, int *nsamples) {
+  if (out) memcpy(out, s->stats, sizeof(double) * (size_t)OR_NSTATS * s->nrm * s->nzm);
+  if (nsamples) *nsamples = s->nstatsamples;
+  return 0;
+}
+
+double or_last_nute(or_state *s) { return s->last_nute; }
+
+/* ------------------------------------------------------------------ */
 /* main.F90:152-223: one iteration of the step loop                    */
 /* ------------------------------------------------------------------ */
 int or_step(or_state *s, int ntime, double walltime, double *o) {
@@ -1704,6 +1825,10 @@ int or_step(or_state *s, int ntime, double walltime, double *o) {
     o[7] = vmax[1];
     o[8] = vmax[2];
     if (fabs(dmax) > s->p.resid) errorcode = 169;
+    if (s->time > s->p.tsta) { /* main.F90:188-190 */
+      or_calc_stats(s);
+      or_calc_dissipation(s);
+    }
   }
   if (s->time > s->p.tmax) errorcode = 333;
   if (walltime > s->p.walltimemax) errorcode = 334;

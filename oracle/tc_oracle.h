@@ -87,6 +87,17 @@ double or_check_divergence(or_state *s);
 void or_calc_torque(or_state *s, double *inner, double *outer);
 void or_calc_velocity_stats(or_state *s, double *vmax3, double *rms3);
 
+/* stat_arrays of StatRoutines.F90 / CalcDissipation.F90, in WriteStats
+ * order; each is (nrm, nzm), Fortran order. */
+enum { OR_ST_VT_ME = 0, OR_ST_VR_ME, OR_ST_VZ_ME, OR_ST_WT_ME, OR_ST_WR_ME, OR_ST_WZ_ME,
+       OR_ST_VT_RMS, OR_ST_VR_RMS, OR_ST_VZ_RMS, OR_ST_WT_RMS, OR_ST_WR_RMS, OR_ST_WZ_RMS,
+       OR_ST_VTVR_ME, OR_ST_VTVZ_ME, OR_ST_VRVZ_ME, OR_ST_PR_ME, OR_ST_PR_RMS, OR_ST_DISSTE,
+       OR_NSTATS };
+void or_calc_stats(or_state *s);          /* CalcStats incl. CalcVorticity */
+double or_calc_dissipation(or_state *s);  /* CalcDissipation; returns nute */
+int or_get_stats(or_state *s, double *out, int *nsamples);
+double or_last_nute(or_state *s);         /* NaN before the first call */
+
 /* Initial-field projection used to build divergence-free starting fields:
  * CalcLocalDivergence + SolvePressureCorrection + CorrectVelocity with
  * al = dt = 1, pr untouched. */

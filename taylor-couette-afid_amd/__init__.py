@@ -61,7 +61,14 @@ ABI_SYMBOLS = [
     "tc_batch_create", "tc_batch_destroy", "tc_batch_add", "tc_batch_remove", "tc_batch_ctx",
     "tc_batch_advance", "tc_batch_status", "tc_batch_groups",
     "tc_write_restart", "tc_read_restart",
+    "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
+    "tc_last_dissipation",
 ]
+
+# stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
+STATS = ["vt_mean", "vr_mean", "vz_mean", "wt_mean", "wr_mean", "wz_mean",
+         "vt_rms", "vr_rms", "vz_rms", "wt_rms", "wr_rms", "wz_rms",
+         "vtvr_mean", "vtvz_mean", "vrvz_mean", "pr_mean", "pr_rms", "disste"]
 
 TUNING_FIELDS = ["helm_lpt", "fft_t_lds", "fft_z_lds", "fft_z_nt", "fft_pad", "poisson_pq"]
 
@@ -156,6 +163,11 @@ def lib():
     L.tc_batch_groups.argtypes = [_V, _I, _I]
     L.tc_write_restart.argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
     L.tc_read_restart.argtypes = [_V, ctypes.c_char_p, _I]
+    L.tc_calc_stats.argtypes = [_V]
+    L.tc_calc_dissipation.argtypes = [_V, _D]
+    L.tc_get_stats.argtypes = [_V, _D, _I]
+    L.tc_set_stats.argtypes = [_V, _D, ctypes.c_int]
+    L.tc_last_dissipation.argtypes = [_V, _D]
     _LIB = L
     return L
 
@@ -343,6 +355,37 @@ class Solver:
         n = ctypes.c_int()
         _check(lib().tc_read_restart(self.h, os.fsencode(path), ctypes.byref(n)))
         return n.value
+
+    # --- statistics (StatRoutines.F90, CalcVorticity.F90, CalcDissipation.F90) ---
+    def CalcStats(self):
+        _check(lib().tc_calc_stats(self.h))
+
+    def CalcDissipation(self):
+        """Adds to disste; returns nute."""
+        v = np.zeros(1)
+        _check(lib().tc_calc_dissipation(self.h, _dp(v)))
+        return float(v[0])
+
+    def last_dissipation(self):
+        v = np.zeros(1)
+        _check(lib().tc_last_dissipation(self.h, _dp(v)))
+        return float(v[0])
+
+    def stats(self):
+        """dict name -> (nrm, nzm) accumulator, and the sample count"""
+        m = self.nrm * self.nzm
+        buf = np.zeros(len(STATS) * m)
+        n = ctypes.c_int()
+        _check(lib().tc_get_stats(self.h, _dp(buf), ctypes.byref(n)))
+        return {name: buf[q * m:(q + 1) * m].reshape((self.nrm, self.nzm), order="F")
+                for q, name in enumerate(STATS)}, n.value
+
+    def set_stats(self, stats, nsamples):
+        m = self.nrm * self.nzm
+        buf = np.zeros(len(STATS) * m)
+        for q, name in enumerate(STATS):
+            buf[q * m:(q + 1) * m] = np.asarray(stats[name], dtype=np.float64).reshape(-1, order="F")
+        _check(lib().tc_set_stats(self.h, _dp(buf), int(nsamples)))
 
     def step(self, ntime, walltime=0.0):
         """One main.F90 loop iteration; returns (errorcode, diagnostics)."""

@@ -18,7 +18,8 @@ HOST = os.path.join(HERE, "tc_host")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("TC_FC", "/opt/rocm/bin/amdflang")
-SOURCES = ["tc_grid.cpp", "tc_abi.cpp", "tc_batch.cpp", "tc_kernels.hip", "tc_poisson.hip"]
+SOURCES = ["tc_grid.cpp", "tc_abi.cpp", "tc_batch.cpp", "tc_kernels.hip", "tc_poisson.hip",
+           "tc_stats.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     # no packed-FP32 instructions anywhere (all arithmetic here is FP64)

@@ -209,6 +209,8 @@ void destroy_ctx(tc_ctx *c) {
   if (c->tw_z) hipFree(c->tw_z);
   if (c->red) hipFree(c->red);
   if (c->red_out) hipFree(c->red_out);
+  if (c->stats) hipFree(c->stats);
+  if (c->stat_part) hipFree(c->stat_part);
   if (c->red_host) hipHostFree(c->red_host);
   for (auto &t : c->timers) {
     hipEventDestroy(t.a);
@@ -430,7 +432,8 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
                                        &g.udx2m, &g.udx2c, &g.rudx2c, &g.udx1vis1, &g.udx1vis2,
                                        &g.udx1qm, &g.pvol, &g.ap1j,  &g.am1j,  &g.ac1j,
                                        &g.ap2je, &g.am2je, &g.ac2je, &g.ap3j,  &g.am3j,
-                                       &g.ac3j,  &g.apph,  &g.amph,  &g.acph,  &g.ak1, &g.ak3};
+                                       &g.ac3j,  &g.apph,  &g.amph,  &g.acph,  &g.ak1, &g.ak3,
+                                       &g.udx2mm, &g.udx2cc};
     const int nv = (int)(sizeof vs / sizeof vs[0]);
     size_t stride = (size_t)std::max(c->nr, std::max(c->nthm, c->nzm)) + 2;
     stride = (stride + 7) / 8 * 8;  // keeps the plane records 64-byte aligned
@@ -473,7 +476,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
                             &c->gd.udx1vis2, &c->gd.udx1qm, &c->gd.pvol, &c->gd.ap1j,
                             &c->gd.am1j, &c->gd.ac1j, &c->gd.ap2je, &c->gd.am2je, &c->gd.ac2je,
                             &c->gd.ap3j, &c->gd.am3j, &c->gd.ac3j, &c->gd.apph, &c->gd.amph,
-                            &c->gd.acph, &c->gd.ak1, &c->gd.ak3};
+                            &c->gd.acph, &c->gd.ak1, &c->gd.ak3, &c->gd.udx2mm, &c->gd.udx2cc};
     for (int q = 0; q < nv; q++) *dst[q] = c->gvbuf + q * stride;
     c->gd.hp = reinterpret_cast<const tc::HelmPlane *>(c->gvbuf + stride * nv);
     c->gd.dxt = g.dxt;
@@ -530,6 +533,14 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   TRY(dev_alloc(c, (void **)&c->red, sizeof(double) * 16 * (size_t)((c->P + 255) / 256)));
   TRY(dev_alloc(c, (void **)&c->red_out, sizeof(double) * 16));
   HTRY(hipHostMalloc((void **)&c->red_host, sizeof(double) * 16, hipHostMallocDefault));
+  // statistics accumulators (StatRoutines.F90 InitStats: zero)
+  {
+    const size_t m = (size_t)c->nrm * c->nzm;
+    TRY(dev_alloc(c, (void **)&c->stats, sizeof(double) * TC_NSTATS * m));
+    TRY(dev_alloc(c, (void **)&c->stat_part, sizeof(double) * m));
+    HTRY(hipMemsetAsync(c->stats, 0, sizeof(double) * TC_NSTATS * m, c->stream));
+    c->last_nute = std::nan("");
+  }
   set_substep(c, 1);
   c->helm_lpt = t.helm_lpt ? t.helm_lpt : tc::choose_helm_lpt(c);
   c->poisson_pq = t.poisson_pq;
@@ -947,6 +958,8 @@ int tc_calc_velocity_stats(tc_ctx *c, double *vmax3, double *rms3) {
   return 0;
 }
 
+static int run_stats(tc_ctx *c, bool stats, bool diss);
+
 int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
   CTXCHK(c);
   // main.F90:156-223
@@ -996,6 +1009,10 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
     c->cfl_cached = out[0];
     c->cfl_valid = true;
     if (std::fabs(out[3]) > p.resid) errorcode = 169;
+    if (c->time > p.tsta) {  // main.F90:188-190
+      rc = run_stats(c, true, true);
+      if (rc) return rc;
+    }
   }
   if (c->time > p.tmax) errorcode = 333;
   if (walltime > p.walltimemax) errorcode = 334;
@@ -1003,6 +1020,69 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
   if (o)
     for (int q = 0; q < 9; q++) o[q] = out[q];
   return errorcode;
+}
+
+// One statistics pass; with diss, nute = volume sum / (nthm nzm volt
+// lamdissip) as CalcDissipation.F90:142-148 forms it.
+static int run_stats(tc_ctx *c, bool stats, bool diss) {
+  tc::launch_stats(c, stats, diss);
+  int rc = launch_status();
+  if (rc) return rc;
+  if (stats) c->nstatsamples = c->nstatsamples + 1;
+  if (diss) {
+    double sum;
+    rc = read_reduction(c, 1, &sum);
+    if (rc) return rc;
+    const tc_params &p = c->p;
+    const double pi = c->g.pi;
+    const double volt = pi * (p.rext * p.rext - p.rint * p.rint);
+    const double w = (p.virotint / p.rint - p.virotext / p.rext) / (p.rext - p.rint);
+    const double lamdissip =
+        4.0 * (p.rext * p.rext) * (p.rint * p.rint) / ((p.rext + p.rint) * (p.rext + p.rint)) * (w * w);
+    c->last_nute = sum / ((double)c->nthm * (double)c->nzm * volt * lamdissip);
+  }
+  return 0;
+}
+
+int tc_calc_stats(tc_ctx *c) {
+  CTXCHK(c);
+  return run_stats(c, true, false);
+}
+
+int tc_calc_dissipation(tc_ctx *c, double *nute) {
+  CTXCHK(c);
+  const int rc = run_stats(c, false, true);
+  if (rc) return rc;
+  if (nute) *nute = c->last_nute;
+  return 0;
+}
+
+int tc_get_stats(tc_ctx *c, double *out, int *nsamples) {
+  CTXCHK(c);
+  if (out) {
+    HIPCHK(hipMemcpyAsync(out, c->stats, sizeof(double) * TC_NSTATS * (size_t)c->nrm * c->nzm,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (nsamples) *nsamples = c->nstatsamples;
+  return 0;
+}
+
+int tc_set_stats(tc_ctx *c, const double *in, int nsamples) {
+  CTXCHK(c);
+  if (!in) return fail("null stats array");
+  if (nsamples < 0) return fail("negative sample count");
+  HIPCHK(hipMemcpyAsync(c->stats, in, sizeof(double) * TC_NSTATS * (size_t)c->nrm * c->nzm,
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nstatsamples = nsamples;
+  return 0;
+}
+
+int tc_last_dissipation(tc_ctx *c, double *nute) {
+  CTXCHK(c);
+  if (nute) *nute = c->last_nute;
+  return 0;
 }
 
 int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed, double amp, double *qt,

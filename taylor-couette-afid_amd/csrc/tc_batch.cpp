@@ -339,6 +339,9 @@ int tc_batch_add(tc_batch *b, const tc_params *p, const tc_tuning *t, int *id) {
   if (p->minstrt * p->multmin + p->minstrz * p->multmin != 0)
     return tc::set_error("batched instances need no-slip walls everywhere (MINSTRT*MULTMIN = MINSTRZ*MULTMIN = 0)");
   if (!(p->dtmax > 0.0)) return tc::set_error("batched instances need DTMAX > 0");
+  if (p->tsta < p->tmax)
+    return tc::set_error("batched instances keep no statistics: TSTA must not be below TMAX "
+                         "(run statistics on a single instance, tc_step)");
   tc_ctx *c = nullptr;
   int rc = tc_create_tuned(p, t, &c);
   if (rc) return rc;

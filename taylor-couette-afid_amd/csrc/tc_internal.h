@@ -38,6 +38,7 @@ struct GridDev {
       *udx1vis2, *udx1qm, *pvol;
   const double *ap1j, *am1j, *ac1j, *ap2je, *am2je, *ac2je, *ap3j, *am3j, *ac3j;
   const double *apph, *amph, *acph, *ak1, *ak3;
+  const double *udx2mm, *udx2cc;  // statistics and dissipation only
   double dxt, dxz, dxtq, dxzq, ren, romeg;
   double ap1j_fs2, ac1j_fs1, ac1j_fs2, am1j_fs1, ap3j_fs2, ac3j_fs1, ac3j_fs2, am3j_fs1;
   int nthm, nzm, nrm, nr, nth2, sstr;  // sstr: spectral row stride >= nth2
@@ -150,6 +151,10 @@ struct tc_ctx {
   int poisson_pq = 13;  // rows per thread of the radial Poisson solve (13 or 26)
   int helm_lpt = 1;  // radial lines per thread in the fused Helmholtz kernel (1 or 2)
   tc_tuning tuning{};  // effective tuning (helm_lpt 0 = chosen by occupancy)
+  // statistics accumulators (TC_NSTATS x nrm x nzm) and per-line partials
+  double *stats = nullptr, *stat_part = nullptr;
+  int nstatsamples = 0;
+  double last_nute = 0.0;  // NaN until the first CalcDissipation
   // CFL of the current velocity, when known (set by the logging pass of tc_step)
   bool cfl_valid = false;
   double cfl_cached = 0.0;
@@ -184,6 +189,7 @@ void launch_torque(tc_ctx *c);
 void launch_velocity_stats(tc_ctx *c);
 // one pass: cfl, max|div|, vmax(3), rms sums(3), torque sums(2)
 void launch_diagnostics(tc_ctx *c);
+void launch_stats(tc_ctx *c, bool stats, bool diss);
 void launch_copy(tc_ctx *c, int dst, int src);
 
 // grouped launches over n instances of one shape (c: any instance of the

@@ -19,7 +19,10 @@
 !   --checkpoint F  write a checkpoint of the final state
 ! Outputs: torquenu.out (time, inner, outer; e14.6 as CalcTorque.F90:59),
 !          torquenu_full.out (same at full precision), vmax.out,
-!          continua_{qt,qr,qz,pr}.bin at exit (raw FP64, (nr,nzm,nthm)).
+!          dissipnu.out (time, nute once time > TSTA),
+!          continua_{qt,qr,qz,pr}.bin at exit (raw FP64, (nr,nzm,nthm)),
+!          stats/<name>.bin (raw FP64, (nrm,nzm)) and
+!          stats/averaging_time.txt on a normal exit.
 !-----------------------------------------------------------------------
 module tc_abi
   use iso_c_binding
@@ -172,6 +175,21 @@ module tc_abi
     integer(c_int) function tc_synchronize(ctx) bind(C, name='tc_synchronize')
       import
       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function tc_calc_stats(ctx) bind(C, name='tc_calc_stats')
+      import
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function tc_calc_dissipation(ctx, nute) bind(C, name='tc_calc_dissipation')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: nute
+    end function
+    integer(c_int) function tc_get_stats(ctx, a, nsamples) bind(C, name='tc_get_stats')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: a(*)
+      integer(c_int), intent(out) :: nsamples
     end function
     integer(c_int) function tc_get_scalars(ctx, out8) bind(C, name='tc_get_scalars')
       import
@@ -441,6 +459,68 @@ subroutine CalcTorque
   write(96, '(3(1x,es24.16))') time, anusslow, anussup
 end subroutine
 
+! StatRoutines.F90:12-71 (with CalcVorticity) on the device
+subroutine CalcStats
+  use param
+  use tc_abi
+  call tc_check(tc_calc_stats(ctx), 'CalcStats')
+end subroutine
+! CalcDissipation.F90:11-153 on the device; dissipnu.out as :150
+subroutine CalcDissipation
+  use param
+  use tc_abi
+  real :: nute
+  call tc_check(tc_calc_dissipation(ctx, nute), 'CalcDissipation')
+  write(92, *) time, nute
+end subroutine
+! StatRoutines.F90:73-171: one raw (nrm, nzm) file per accumulator under
+! stats/, added to the existing files when STAREA /= 0
+! (StatReadReduceWrite.F90:27-31), and the sample count in
+! stats/averaging_time.txt
+subroutine WriteStats
+  use param
+  use tc_abi
+  implicit none
+  character(len=9), parameter :: names(18) = [character(len=9) :: 'vt_mean', 'vr_mean', &
+      'vz_mean', 'wt_mean', 'wr_mean', 'wz_mean', 'vt_rms', 'vr_rms', 'vz_rms', 'wt_rms', &
+      'wr_rms', 'wz_rms', 'vtvr_mean', 'vtvz_mean', 'vrvz_mean', 'pr_mean', 'pr_rms', 'disste']
+  real, allocatable :: a(:,:,:), old(:,:)
+  integer(c_int) :: ns
+  integer :: q, u, ios, nold
+  allocate(a(nrm, nzm, 18), old(nrm, nzm))
+  call tc_check(tc_get_stats(ctx, a, ns), 'WriteStats')
+  nold = 0
+  if (bou%starea /= 0) then
+    open(newunit=u, file='stats/averaging_time.txt', status='old', iostat=ios)
+    if (ios == 0) then
+      read(u, *) nold
+      close(u)
+    end if
+  end if
+  do q = 1, 18
+    if (bou%starea /= 0) then
+      open(newunit=u, file='stats/'//trim(names(q))//'.bin', access='stream', &
+           form='unformatted', status='old', iostat=ios)
+      if (ios == 0) then
+        read(u) old
+        close(u)
+        a(:, :, q) = a(:, :, q) + old
+      end if
+    end if
+    open(newunit=u, file='stats/'//trim(names(q))//'.bin', access='stream', &
+         form='unformatted', status='replace', iostat=ios)
+    if (ios /= 0) then
+      write(6, *) 'WriteStats: cannot write stats/ (create the directory first)'
+      return
+    end if
+    write(u) a(:, :, q)
+    close(u)
+  end do
+  open(newunit=u, file='stats/averaging_time.txt', status='replace')
+  write(u, *) ns + nold
+  close(u)
+end subroutine
+
 ! Whole-state checkpoint of the library (fields, pressure, RK histories,
 ! time, step index): the run continues bit for bit from step ntime0
 subroutine RestartFromCheckpoint
@@ -526,6 +606,7 @@ program tc_host
   open(97, file='torquenu.out', status='replace')
   open(96, file='torquenu_full.out', status='replace')
   open(95, file='vmax.out', status='replace')
+  open(92, file='dissipnu.out', status='replace')
 
   time = 0.d0
   if (len_trim(restart_file) > 0) then
@@ -573,6 +654,10 @@ program tc_host
       call CheckDivergence(dmax)
       call CalcTorque
       if (abs(dmax) > resid) errorcode = 169
+      if (time > bou%tsta) then  ! main.F90:188-190
+        call CalcStats
+        call CalcDissipation
+      end if
     end if
 
     if (time > tmax) errorcode = 333
@@ -586,6 +671,8 @@ program tc_host
   call system_clock(c1)
   write(6, '(a,i0,a,i0,a,es12.5,a,f10.3,a)') 'exit code ', errorcode, ' after step ', ntime, &
       ' time ', time, ' wall ', dble(c1 - c0) / dble(crate), ' s'
+  ! QuitRoutine.F90:28-30: statistics and fields on a normal exit
+  if (errorcode == 333 .or. errorcode == 334 .or. errorcode == 555) call WriteStats
   call WriteFlowField
   if (len_trim(checkpoint_file) > 0) then
     ! the device holds the start time of the last step; store the time reached
@@ -593,7 +680,7 @@ program tc_host
     call tc_check(tc_write_restart(ctx, trim(checkpoint_file)//c_null_char, int(ntime + 1, c_int)), &
                   'checkpoint')
   end if
-  close(95); close(96); close(97)
+  close(92); close(95); close(96); close(97)
   call tc_check(tc_destroy(ctx), 'tc_destroy')
   if (errorcode /= 333 .and. errorcode /= 334 .and. errorcode /= 555 .and. errorcode /= 0) stop 3
 end program tc_host

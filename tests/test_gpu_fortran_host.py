@@ -127,3 +127,41 @@ def test_fortran_host_checkpoint_restart(tc, tmp_path):
     for n in ("qt", "qr", "qz", "pr"):
         got = np.fromfile(part / f"continua_{n}.bin").reshape(shape, order="F")
         assert np.array_equal(got, f_full[n]), n
+
+
+def test_fortran_host_statistics(tc, oracle, tmp_path):
+    """TSTA inside the run: CalcStats and CalcDissipation at every logging
+    step after it, dissipnu.out, and stats/*.bin written on the normal exit
+    match the oracle's accumulators."""
+    if not os.path.exists(HOST):
+        pytest.fail("tc_host is not built")
+    p = _params(tc)
+    p.update(tsta=0.004)
+    (tmp_path / "stats").mkdir()
+    write_bou(tmp_path / "bou.in", p)
+    r = subprocess.run([HOST, "--ic", "perturbed", "77", "0.01", "--fused"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "exit code 555" in r.stdout, r.stdout + r.stderr
+    dev = tc.Solver(p)
+    qt, qr, qz = tc.init_perturbed_couette(p, 77, 0.01)
+    dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+    dev.project()
+    o = oracle.Oracle(p)
+    for n in ("qt", "qr", "qz"):
+        o.set(n, dev.get(n))
+    dev.close()
+    nute = []
+    for n in range(p["ntst"] + 1):
+        code, d = o.step(n)
+        if d["time"] > p["tsta"]:
+            nute.append((d["time"], o.last_nute()))
+    assert code == 555 and len(nute) >= 2
+    got = np.loadtxt(tmp_path / "dissipnu.out", ndmin=2)
+    assert got.shape == (len(nute), 2)
+    np.testing.assert_allclose(got, np.array(nute), rtol=1e-9)
+    want, ns = o.stats()
+    assert int(open(tmp_path / "stats" / "averaging_time.txt").read().split()[0]) == ns
+    for name, w in want.items():
+        a = np.fromfile(tmp_path / "stats" / f"{name}.bin").reshape(w.shape, order="F")
+        scale = np.abs(w).max()
+        assert np.abs(a - w).max() <= 1e-9 * (scale if scale > 0 else 1.0), name

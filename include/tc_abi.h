@@ -238,6 +238,20 @@ int tc_set_stats(tc_ctx *ctx, const double *in, int nsamples);
  * of tc_step once time > TSTA, main.F90:188-190); NaN before the first. */
 int tc_last_dissipation(tc_ctx *ctx, double *nute);
 
+/* ---------------- slab dumps ---------------- */
+
+/* DumpRCuts.F90:28-36 at r-plane kslab (1..nrm): qt, the cell-centre mean
+ * of the radial velocity (qr/r on the two faces), qz; each (nzm, nthm) in
+ * Fortran order.  InitDumpRCuts.F90 picks kslab from rcuts.in; that
+ * selection is host work. */
+int tc_dump_rcut(tc_ctx *ctx, int kslab, double *vt, double *vr, double *vz);
+/* DumpThCuts.F90:33-44: the theta index 1 cut of qt, qr/rc, qz; each
+ * (nrm, nzm) in Fortran order. */
+int tc_dump_thcut(tc_ctx *ctx, double *vt, double *vr, double *vz);
+/* DumpStreamwiseAvg.F90:33-44: theta-means of qt, qr/rc, qz; each
+ * (nrm, nzm) in Fortran order. */
+int tc_dump_streamwise_avg(tc_ctx *ctx, double *vt, double *vr, double *vz);
+
 /* ---------------- batched sweep (no counterpart in the reference) ---------------- */
 
 /* Many independent instances, each a tc_ctx of its own shape and

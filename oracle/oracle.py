@@ -98,6 +98,9 @@ def lib():
         L.or_get_stats.argtypes = [ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int)]
         L.or_last_nute.restype = ctypes.c_double
         L.or_last_nute.argtypes = [ctypes.c_void_p]
+        L.or_dump_rcut.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D]
+        L.or_dump_thcut.argtypes = [ctypes.c_void_p, _D, _D, _D]
+        L.or_dump_streamwise_avg.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.or_step.restype = ctypes.c_int
         L.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]
         for n in ["or_dgttrf", "or_zgttrf"]:
@@ -228,6 +231,22 @@ class Oracle:
             out[q] = buf[q * self.nrm * self.nzm:(q + 1) * self.nrm * self.nzm].reshape(
                 (self.nrm, self.nzm), order="F")
         return {name: out[q] for q, name in enumerate(STATS)}, n.value
+
+    def dump_rcut(self, kslab):
+        """vt, vr, vz at r-plane kslab, each (nzm, nthm)"""
+        out = [np.zeros((self.nzm, self.nthm), order="F") for _ in range(3)]
+        lib().or_dump_rcut(self.h, int(kslab), *[_dp(a) for a in out])
+        return out
+
+    def dump_thcut(self):
+        out = [np.zeros((self.nrm, self.nzm), order="F") for _ in range(3)]
+        lib().or_dump_thcut(self.h, *[_dp(a) for a in out])
+        return out
+
+    def dump_streamwise_avg(self):
+        out = [np.zeros((self.nrm, self.nzm), order="F") for _ in range(3)]
+        lib().or_dump_streamwise_avg(self.h, *[_dp(a) for a in out])
+        return out
 
     def step(self, ntime, walltime=0.0):
         o = np.zeros(9)

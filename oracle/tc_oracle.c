@@ -1785,6 +1785,51 @@ int or_get_stats(or_state *s, double *out, int *nsamples) {
 
 double or_last_nute(or_state *s) { return s->last_nute; }
 
+/* DumpRCuts.F90:28-36; outputs (nzm, nthm), Fortran order */
+void or_dump_rcut(or_state *s, int kslab, double *vt, double *vr, double *vz) {
+  const int nzm = s->nzm;
+  for (int i = 1; i <= s->nthm; i++)
+    for (int j = 1; j <= nzm; j++) {
+      const size_t t = (size_t)(j - 1) + (size_t)nzm * (i - 1);
+      vz[t] = QZ(kslab, j, i);
+      vt[t] = QT(kslab, j, i);
+      vr[t] = (QR(kslab, j, i) * s->usrc[kslab] + QR(kslab + 1, j, i) * s->usrc[kslab + 1]) * 0.5;
+    }
+}
+
+/* DumpThCuts.F90:29-44; outputs (nrm, nzm), Fortran order */
+void or_dump_thcut(or_state *s, double *vt, double *vr, double *vz) {
+  const int nrm = s->nrm;
+  const size_t m = (size_t)nrm * s->nzm;
+  for (size_t t = 0; t < m; t++) vt[t] = vr[t] = vz[t] = 0.0;
+  for (int i = 1; i <= s->nthm; i++)
+    for (int j = 1; j <= s->nzm; j++)
+      for (int k = 1; k <= nrm; k++) {
+        if (i == 1) {
+          const size_t t = (size_t)(k - 1) + (size_t)nrm * (j - 1);
+          vt[t] = vt[t] + QT(k, j, i);
+          vr[t] = vr[t] + QR(k, j, i) * s->usrc[k];
+          vz[t] = vz[t] + QZ(k, j, i);
+        }
+      }
+}
+
+/* DumpStreamwiseAvg.F90:29-44; outputs (nrm, nzm), Fortran order */
+void or_dump_streamwise_avg(or_state *s, double *vt, double *vr, double *vz) {
+  const int nrm = s->nrm;
+  const size_t m = (size_t)nrm * s->nzm;
+  const double usnthm = 1.0 / (double)s->nthm;
+  for (size_t t = 0; t < m; t++) vt[t] = vr[t] = vz[t] = 0.0;
+  for (int i = 1; i <= s->nthm; i++)
+    for (int j = 1; j <= s->nzm; j++)
+      for (int k = 1; k <= nrm; k++) {
+        const size_t t = (size_t)(k - 1) + (size_t)nrm * (j - 1);
+        vt[t] = vt[t] + QT(k, j, i) * usnthm;
+        vr[t] = vr[t] + QR(k, j, i) * s->usrc[k] * usnthm;
+        vz[t] = vz[t] + QZ(k, j, i) * usnthm;
+      }
+}
+
 /* ------------------------------------------------------------------ */
 /* main.F90:152-223: one iteration of the step loop                    */
 /* ------------------------------------------------------------------ */

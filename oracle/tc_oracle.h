@@ -97,6 +97,10 @@ void or_calc_stats(or_state *s);          /* CalcStats incl. CalcVorticity */
 double or_calc_dissipation(or_state *s);  /* CalcDissipation; returns nute */
 int or_get_stats(or_state *s, double *out, int *nsamples);
 double or_last_nute(or_state *s);         /* NaN before the first call */
+/* slab dumps (DumpRCuts, DumpThCuts, DumpStreamwiseAvg) */
+void or_dump_rcut(or_state *s, int kslab, double *vt, double *vr, double *vz);
+void or_dump_thcut(or_state *s, double *vt, double *vr, double *vz);
+void or_dump_streamwise_avg(or_state *s, double *vt, double *vr, double *vz);
 
 /* Initial-field projection used to build divergence-free starting fields:
  * CalcLocalDivergence + SolvePressureCorrection + CorrectVelocity with

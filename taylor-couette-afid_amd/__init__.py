@@ -62,7 +62,7 @@ ABI_SYMBOLS = [
     "tc_batch_advance", "tc_batch_status", "tc_batch_groups",
     "tc_write_restart", "tc_read_restart",
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
-    "tc_last_dissipation",
+    "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -168,6 +168,9 @@ def lib():
     L.tc_get_stats.argtypes = [_V, _D, _I]
     L.tc_set_stats.argtypes = [_V, _D, ctypes.c_int]
     L.tc_last_dissipation.argtypes = [_V, _D]
+    L.tc_dump_rcut.argtypes = [_V, ctypes.c_int, _D, _D, _D]
+    L.tc_dump_thcut.argtypes = [_V, _D, _D, _D]
+    L.tc_dump_streamwise_avg.argtypes = [_V, _D, _D, _D]
     _LIB = L
     return L
 
@@ -386,6 +389,23 @@ class Solver:
         for q, name in enumerate(STATS):
             buf[q * m:(q + 1) * m] = np.asarray(stats[name], dtype=np.float64).reshape(-1, order="F")
         _check(lib().tc_set_stats(self.h, _dp(buf), int(nsamples)))
+
+    # --- slab dumps (DumpRCuts.F90, DumpThCuts.F90, DumpStreamwiseAvg.F90) ---
+    def DumpRCut(self, kslab):
+        """vt, vr, vz at r-plane kslab (1..nrm), each (nzm, nthm)"""
+        out = [np.zeros((self.nzm, self.nthm), order="F") for _ in range(3)]
+        _check(lib().tc_dump_rcut(self.h, int(kslab), *[_dp(a) for a in out]))
+        return out
+
+    def DumpThCut(self):
+        out = [np.zeros((self.nrm, self.nzm), order="F") for _ in range(3)]
+        _check(lib().tc_dump_thcut(self.h, *[_dp(a) for a in out]))
+        return out
+
+    def DumpStreamwiseAvg(self):
+        out = [np.zeros((self.nrm, self.nzm), order="F") for _ in range(3)]
+        _check(lib().tc_dump_streamwise_avg(self.h, *[_dp(a) for a in out]))
+        return out
 
     def step(self, ntime, walltime=0.0):
         """One main.F90 loop iteration; returns (errorcode, diagnostics)."""

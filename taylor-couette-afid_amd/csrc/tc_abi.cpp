@@ -211,6 +211,7 @@ void destroy_ctx(tc_ctx *c) {
   if (c->red_out) hipFree(c->red_out);
   if (c->stats) hipFree(c->stats);
   if (c->stat_part) hipFree(c->stat_part);
+  if (c->dump) hipFree(c->dump);
   if (c->red_host) hipHostFree(c->red_host);
   for (auto &t : c->timers) {
     hipEventDestroy(t.a);
@@ -1077,6 +1078,40 @@ int tc_set_stats(tc_ctx *c, const double *in, int nsamples) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->nstatsamples = nsamples;
   return 0;
+}
+
+// Slab dumps: extraction on the device, three arrays of n doubles out.
+static int run_dump(tc_ctx *c, int kind, int k, double *vt, double *vr, double *vz, size_t n) {
+  if (!vt || !vr || !vz) return fail("null output array");
+  if (!c->dump) {
+    const size_t m = std::max((size_t)c->P, (size_t)c->nrm * c->nzm);
+    if (dev_alloc(c, (void **)&c->dump, sizeof(double) * 3 * m)) return fail("out of device memory");
+  }
+  tc::launch_dump(c, kind, k);
+  int rc = launch_status();
+  if (rc) return rc;
+  double *outs[3] = {vt, vr, vz};
+  for (int q = 0; q < 3; q++)
+    HIPCHK(hipMemcpyAsync(outs[q], c->dump + q * n, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tc_dump_rcut(tc_ctx *c, int kslab, double *vt, double *vr, double *vz) {
+  CTXCHK(c);
+  if (kslab < 1 || kslab > c->nrm) return fail("kslab out of 1..nrm");
+  return run_dump(c, 0, kslab, vt, vr, vz, (size_t)c->P);
+}
+
+int tc_dump_thcut(tc_ctx *c, double *vt, double *vr, double *vz) {
+  CTXCHK(c);
+  return run_dump(c, 1, 0, vt, vr, vz, (size_t)c->nrm * c->nzm);
+}
+
+int tc_dump_streamwise_avg(tc_ctx *c, double *vt, double *vr, double *vz) {
+  CTXCHK(c);
+  return run_dump(c, 2, 0, vt, vr, vz, (size_t)c->nrm * c->nzm);
 }
 
 int tc_last_dissipation(tc_ctx *c, double *nute) {

@@ -155,6 +155,7 @@ struct tc_ctx {
   double *stats = nullptr, *stat_part = nullptr;
   int nstatsamples = 0;
   double last_nute = 0.0;  // NaN until the first CalcDissipation
+  double *dump = nullptr;  // slab extraction, 3 x max(P, nrm*nzm), made at first use
   // CFL of the current velocity, when known (set by the logging pass of tc_step)
   bool cfl_valid = false;
   double cfl_cached = 0.0;
@@ -190,6 +191,7 @@ void launch_velocity_stats(tc_ctx *c);
 // one pass: cfl, max|div|, vmax(3), rms sums(3), torque sums(2)
 void launch_diagnostics(tc_ctx *c);
 void launch_stats(tc_ctx *c, bool stats, bool diss);
+void launch_dump(tc_ctx *c, int kind, int k);
 void launch_copy(tc_ctx *c, int dst, int src);
 
 // grouped launches over n instances of one shape (c: any instance of the

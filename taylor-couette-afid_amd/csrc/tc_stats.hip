@@ -149,7 +149,74 @@ __global__ __launch_bounds__(256) void k_sum_fixed(const double *__restrict__ pa
   if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// DumpRCuts.F90:28-36 at r-plane k: vt, vr (mean of the two faces), vz;
+// out[q] is (nzm, nthm) in Fortran order (z fastest).
+__global__ __launch_bounds__(256) void k_rcut(GridDev g, Fields f, int k, double *__restrict__ out) {
+  const long long l = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (l >= g.P) return;
+  const int j = (int)(l / g.nthm), i = (int)(l - (long long)j * g.nthm);
+  const long long o = (long long)(k - 1) * g.P + l;
+  const long long t = (long long)i * g.nzm + j;
+  out[t] = f.qt[o];
+  out[g.P + t] = (f.qr[o] * g.usrc[k] + f.qr[o + g.P] * g.usrc[k + 1]) * 0.5;
+  out[2 * g.P + t] = f.qz[o];
+}
+
+// DumpThCuts.F90:33-44: the theta = 1 cut; out[q] is (nrm, nzm), r fastest.
+__global__ __launch_bounds__(256) void k_thcut(GridDev g, Fields f, double *__restrict__ out) {
+  const long long m = (long long)g.nrm * g.nzm;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= m) return;
+  const int j = (int)(t / g.nrm), k = (int)(t - (long long)j * g.nrm) + 1;
+  const long long o = (long long)(k - 1) * g.P + (long long)j * g.nthm;
+  out[t] = f.qt[o];
+  out[m + t] = f.qr[o] * g.usrc[k];
+  out[2 * m + t] = f.qz[o];
+}
+
+// DumpStreamwiseAvg.F90:33-44: theta-means of vt, vr, vz, one wave per
+// (k, j) line as in k_stats; out[q] is (nrm, nzm), r fastest.
+__global__ __launch_bounds__(256) void k_theta_avg(GridDev g, Fields f, double *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.y + 1;
+  if (j >= g.nzm) return;
+  const double usnthm = 1.0 / (double)g.nthm, usrc = g.usrc[k];
+  const long long row = (long long)(k - 1) * g.P + (long long)j * g.nthm;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = lane; i < g.nthm; i += 64) {
+    s[0] += f.qt[row + i] * usnthm;
+    s[1] += f.qr[row + i] * usrc * usnthm;
+    s[2] += f.qz[row + i] * usnthm;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    double v = s[q];
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) v += __shfl_xor(v, w, 64);
+    s[q] = v;
+  }
+  const long long m = (long long)g.nrm * g.nzm;
+  if (lane < 3) out[lane * m + (long long)(k - 1) + (long long)g.nrm * j] = s[lane];
+}
+
 }  // namespace
+
+// Slab extraction into c->dump (3 arrays); kind 0: r-cut at plane k,
+// 1: theta cut, 2: theta-means.
+void launch_dump(tc_ctx *c, int kind, int k) {
+  Fields f{c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR]};
+  const long long m = (long long)c->nrm * c->nzm;
+  if (kind == 0)
+    hipLaunchKernelGGL(k_rcut, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream, c->gd,
+                       f, k, c->dump);
+  else if (kind == 1)
+    hipLaunchKernelGGL(k_thcut, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream, c->gd, f,
+                       c->dump);
+  else
+    hipLaunchKernelGGL(k_theta_avg, dim3((unsigned)((c->nzm + 3) / 4), (unsigned)c->nrm), dim3(256),
+                       0, c->stream, c->gd, f, c->dump);
+}
 
 // Statistics pass over the current fields.  stats: CalcStats (with the
 // vorticity); diss: CalcDissipation, whose volume sum lands in c->red_out[0].

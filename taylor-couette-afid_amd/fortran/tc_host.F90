@@ -22,7 +22,9 @@
 !          dissipnu.out (time, nute once time > TSTA),
 !          continua_{qt,qr,qz,pr}.bin at exit (raw FP64, (nr,nzm,nthm)),
 !          stats/<name>.bin (raw FP64, (nrm,nzm)) and
-!          stats/averaging_time.txt on a normal exit.
+!          stats/averaging_time.txt on a normal exit; once time > TSTA the
+!          slab dumps rcuts/, thcut/, stave/ every TFRAME2D and fullff/
+!          every TFRAME3D (raw FP64, file names as the reference's .h5).
 !-----------------------------------------------------------------------
 module tc_abi
   use iso_c_binding
@@ -191,6 +193,23 @@ module tc_abi
       real(c_double), intent(out) :: a(*)
       integer(c_int), intent(out) :: nsamples
     end function
+    integer(c_int) function tc_dump_rcut(ctx, k, vt, vr, vz) bind(C, name='tc_dump_rcut')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      real(c_double), intent(out) :: vt(*), vr(*), vz(*)
+    end function
+    integer(c_int) function tc_dump_thcut(ctx, vt, vr, vz) bind(C, name='tc_dump_thcut')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: vt(*), vr(*), vz(*)
+    end function
+    integer(c_int) function tc_dump_streamwise_avg(ctx, vt, vr, vz) &
+        bind(C, name='tc_dump_streamwise_avg')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: vt(*), vr(*), vz(*)
+    end function
     integer(c_int) function tc_get_scalars(ctx, out8) bind(C, name='tc_get_scalars')
       import
       type(c_ptr), value :: ctx
@@ -244,6 +263,7 @@ module param
   logical :: perturbed_ic = .false.
   character(len=512) :: restart_file = '', checkpoint_file = ''
   integer :: ntime0 = 0
+  integer :: nslab = 0, kslab(9) = 1   ! DumpRCuts planes (NSLABR, InitDumpRCuts)
   integer(c_long_long) :: ic_seed = 1
   real :: ic_amp = 1.d-2
   real :: vmax(3)
@@ -521,6 +541,157 @@ subroutine WriteStats
   close(u)
 end subroutine
 
+! ---- slab dumps (raw FP64 files in place of the reference's HDF5) ----
+
+! time stamp of the dump files, DumpRCuts.F90:17-19
+function dump_stamp() result(ipfi)
+  use param
+  implicit none
+  character(len=8) :: ipfi
+  write(ipfi, '(i8.8)') nint(1000. * time + 0.005)
+end function
+
+subroutine write_raw(name, a, n)
+  implicit none
+  character(len=*), intent(in) :: name
+  integer, intent(in) :: n
+  real, intent(in) :: a(n)
+  integer :: u, ios
+  open(newunit=u, file=name, access='stream', form='unformatted', status='replace', iostat=ios)
+  if (ios /= 0) then
+    write(6, *) 'cannot write ', trim(name), ' (create the directory first)'
+    return
+  end if
+  write(u) a
+  close(u)
+end subroutine
+
+! InitDumpRCuts (DumpRCuts.F90:61-95): at most 9 radii from rcuts.in; each
+! takes the plane k = 2..nrm with rm(k-1) < r < rm(k) (1 if none)
+subroutine InitDumpRCuts
+  use param
+  use tc_abi
+  implicit none
+  character(len=4) :: dummy
+  real :: rslab(9)
+  real, allocatable :: rm(:)
+  integer :: u, k, j, m
+  if (nslab > 9) nslab = 9
+  open(newunit=u, file='rcuts.in', status='old')
+  read(u, '(a4)') dummy
+  read(u, *) (rslab(j), j = 1, nslab)
+  close(u)
+  allocate(rm(nr))
+  m = tc_grid_vector(bou, TC_V_RM, rm, nr)
+  call tc_check(m, 'tc_grid_vector')
+  kslab = 1
+  do k = 2, nrm
+    do j = 1, nslab
+      if (rm(k) > rslab(j) .and. rm(k - 1) < rslab(j)) kslab(j) = k
+    end do
+  end do
+  open(newunit=u, file='rcuts/rcutslocs.out', status='replace', iostat=k)
+  if (k == 0) then
+    write(u, *) (kslab(j), j = 1, nslab)
+    close(u)
+  end if
+end subroutine
+
+! DumpRCuts.F90:11-57
+subroutine DumpRCuts
+  use param
+  use tc_abi
+  implicit none
+  interface
+    function dump_stamp() result(ipfi)
+      character(len=8) :: ipfi
+    end function
+  end interface
+  real, allocatable :: vt(:), vr(:), vz(:)
+  character(len=8) :: ipfi
+  character(len=1) :: charm
+  integer :: m
+  allocate(vt(nzm * nthm), vr(nzm * nthm), vz(nzm * nthm))
+  ipfi = dump_stamp()
+  do m = 1, nslab
+    call tc_check(tc_dump_rcut(ctx, int(kslab(m), c_int), vt, vr, vz), 'DumpRCuts')
+    write(charm, '(i1.1)') m
+    call write_raw('rcuts/slab'//charm//'vt_'//ipfi//'.bin', vt, nzm * nthm)
+    call write_raw('rcuts/slab'//charm//'vr_'//ipfi//'.bin', vr, nzm * nthm)
+    call write_raw('rcuts/slab'//charm//'vz_'//ipfi//'.bin', vz, nzm * nthm)
+  end do
+end subroutine
+
+! DumpThCuts.F90:11-71 (the reference names the vr file slabvt_ as well;
+! here it is slabvr_)
+subroutine DumpThCuts
+  use param
+  use tc_abi
+  implicit none
+  interface
+    function dump_stamp() result(ipfi)
+      character(len=8) :: ipfi
+    end function
+  end interface
+  real, allocatable :: vt(:), vr(:), vz(:)
+  character(len=8) :: ipfi
+  allocate(vt(nrm * nzm), vr(nrm * nzm), vz(nrm * nzm))
+  call tc_check(tc_dump_thcut(ctx, vt, vr, vz), 'DumpThCuts')
+  ipfi = dump_stamp()
+  call write_raw('thcut/slabvt_'//ipfi//'.bin', vt, nrm * nzm)
+  call write_raw('thcut/slabvr_'//ipfi//'.bin', vr, nrm * nzm)
+  call write_raw('thcut/slabvz_'//ipfi//'.bin', vz, nrm * nzm)
+end subroutine
+
+! DumpStreamwiseAvg.F90:11-71
+subroutine DumpStreamwiseAvg
+  use param
+  use tc_abi
+  implicit none
+  interface
+    function dump_stamp() result(ipfi)
+      character(len=8) :: ipfi
+    end function
+  end interface
+  real, allocatable :: vt(:), vr(:), vz(:)
+  character(len=8) :: ipfi
+  allocate(vt(nrm * nzm), vr(nrm * nzm), vz(nrm * nzm))
+  call tc_check(tc_dump_streamwise_avg(ctx, vt, vr, vz), 'DumpStreamwiseAvg')
+  ipfi = dump_stamp()
+  call write_raw('stave/slabvt_ax_'//ipfi//'.bin', vt, nrm * nzm)
+  call write_raw('stave/slabvr_ax_'//ipfi//'.bin', vr, nrm * nzm)
+  call write_raw('stave/slabvz_ax_'//ipfi//'.bin', vz, nrm * nzm)
+end subroutine
+
+! DumpFlowField.F90:12-50: qt, qr, qz as (nr, nzm, nthm) and the master
+! scalars as text
+subroutine DumpFlowField
+  use param
+  use tc_abi
+  implicit none
+  interface
+    function dump_stamp() result(ipfi)
+      character(len=8) :: ipfi
+    end function
+  end interface
+  real, allocatable :: a(:)
+  character(len=2), parameter :: names(3) = ['qt', 'qr', 'qz']
+  integer(c_int), parameter :: ids(3) = [TC_QT, TC_QR, TC_QZ]
+  character(len=8) :: ipfi
+  integer :: q, u, ios
+  allocate(a(nr * nzm * nthm))
+  ipfi = dump_stamp()
+  do q = 1, 3
+    call tc_check(tc_download_field(ctx, ids(q), a, TC_LAYOUT_FORTRAN), 'DumpFlowField')
+    call write_raw('fullff/'//names(q)//'_'//ipfi//'.bin', a, nr * nzm * nthm)
+  end do
+  open(newunit=u, file='fullff/ms_'//ipfi//'.txt', status='replace', iostat=ios)
+  if (ios == 0) then
+    write(u, *) time, bou%rint, bou%rext, bou%istr, bou%str, bou%alx3, bou%lamb
+    close(u)
+  end if
+end subroutine
+
 ! Whole-state checkpoint of the library (fields, pressure, RK histories,
 ! time, step index): the run continues bit for bit from step ntime0
 subroutine RestartFromCheckpoint
@@ -602,6 +773,8 @@ program tc_host
 
   call ReadInputFile
   call tc_check(tc_create(bou, ctx), 'tc_create')
+  nslab = bou%nslabr
+  if (nslab /= 0) call InitDumpRCuts  ! main.F90:82
   call system_clock(c0, crate)
   open(97, file='torquenu.out', status='replace')
   open(96, file='torquenu_full.out', status='replace')
@@ -654,9 +827,17 @@ program tc_host
       call CheckDivergence(dmax)
       call CalcTorque
       if (abs(dmax) > resid) errorcode = 169
-      if (time > bou%tsta) then  ! main.F90:188-190
+      if (time > bou%tsta) then  ! main.F90:188-199
         call CalcStats
         call CalcDissipation
+        if (mod(time, bou%tframe2d) < dt) then
+          if (nslab /= 0) call DumpRCuts
+          if (bou%dpthcut /= 0) call DumpThCuts
+          if (bou%dpstave /= 0) call DumpStreamwiseAvg
+        end if
+        if (mod(time, bou%tframe3d) < dt) then
+          if (bou%dpfullf /= 0) call DumpFlowField
+        end if
       end if
     end if
 

@@ -165,3 +165,43 @@ def test_fortran_host_statistics(tc, oracle, tmp_path):
         a = np.fromfile(tmp_path / "stats" / f"{name}.bin").reshape(w.shape, order="F")
         scale = np.abs(w).max()
         assert np.abs(a - w).max() <= 1e-9 * (scale if scale > 0 else 1.0), name
+
+
+def test_fortran_host_slab_dumps(tc, oracle, tmp_path):
+    """rcuts.in radii -> kslab planes (InitDumpRCuts), and the r-cut, theta-cut,
+    streamwise-average and full-field dumps of the last step equal the
+    oracle's at the same state; fullff equals the exit fields."""
+    if not os.path.exists(HOST):
+        pytest.fail("tc_host is not built")
+    p = _params(tc)
+    p.update(tsta=0.0, nslabr=2, dpthcut=1, dpstave=1, tframe2d=1e-9, dpfullf=1, tframe3d=1e-9)
+    for d in ("rcuts", "thcut", "stave", "fullff", "stats"):
+        (tmp_path / d).mkdir()
+    o = oracle.Oracle(p)
+    rm = o.vec("rm")[:p["nrm"]]
+    radii = [0.5 * (rm[3] + rm[4]), 0.5 * (rm[10] + rm[11])]   # between planes 4,5 and 11,12
+    (tmp_path / "rcuts.in").write_text("POSITION OF SLABS\n" + " ".join("%.17g" % r for r in radii) + "\n")
+    tq, f = _run_host(tmp_path, p, fused=True)
+    kslab = [int(x) for x in open(tmp_path / "rcuts" / "rcutslocs.out").read().split()]
+    assert kslab == [5, 12]
+    # the oracle at the same final state
+    tq_o, f_o = _oracle_run(tc, oracle, p)
+    stamp = "%08d" % int(np.floor(1000.0 * tq[-1, 0] + 0.005 + 0.5))
+    o2 = oracle.Oracle(p)
+    for n in ("qt", "qr", "qz"):
+        o2.set(n, f_o[n])
+    nzm, nthm, nrm = p["nzm"], p["nthm"], p["nrm"]
+    for m, k in enumerate(kslab, start=1):
+        want = o2.dump_rcut(k)
+        for q, a in zip(("vt", "vr", "vz"), want):
+            got = np.fromfile(tmp_path / "rcuts" / f"slab{m}{q}_{stamp}.bin").reshape((nzm, nthm), order="F")
+            assert np.abs(got - a).max() <= 1e-9 * np.abs(a).max(), (m, q)
+    for sub, fn, names in (("thcut", o2.dump_thcut, ["slabvt_", "slabvr_", "slabvz_"]),
+                           ("stave", o2.dump_streamwise_avg, ["slabvt_ax_", "slabvr_ax_", "slabvz_ax_"])):
+        for nm, a in zip(names, fn()):
+            got = np.fromfile(tmp_path / sub / f"{nm}{stamp}.bin").reshape((nrm, nzm), order="F")
+            assert np.abs(got - a).max() <= 1e-9 * np.abs(a).max(), (sub, nm)
+    shape = (nrm + 1, nzm, nthm)
+    for n in ("qt", "qr", "qz"):
+        got = np.fromfile(tmp_path / "fullff" / f"{n}_{stamp}.bin").reshape(shape, order="F")
+        assert np.array_equal(got, f[n]), n

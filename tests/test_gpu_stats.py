@@ -99,3 +99,21 @@ def test_stats_match_numpy_theta_means(tc):
                        ("pr_mean", pr[:nrm].mean(axis=2)), ("vt_rms", (qt * qt).mean(axis=2))):
         err = np.abs(st[name] - want).max() / np.abs(want).max()
         assert err < 1e-13, (name, err)
+
+
+@pytest.mark.parametrize("shape", [(32, 24, 33), (100, 18, 12)])
+def test_slab_dumps(tc, oracle, shape):
+    nthm, nzm, nrm = shape
+    p = small_params(tc, nthm, nzm, nrm)
+    dev = tc.Solver(p)
+    orc = oracle.Oracle(p)
+    load_both(dev, orc, random_state(p, 17))
+    for k in (1, nrm // 2, nrm):
+        for a, b in zip(dev.DumpRCut(k), orc.dump_rcut(k)):
+            np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-15)
+    for a, b in zip(dev.DumpThCut(), orc.dump_thcut()):
+        np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-15)
+    for a, b in zip(dev.DumpStreamwiseAvg(), orc.dump_streamwise_avg()):
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14)
+    with pytest.raises(tc.TCError):
+        dev.DumpRCut(nrm + 1)

@@ -204,3 +204,32 @@ def test_step_runs_stats_after_tsta(tc, oracle):
         seen.append(o.stats()[1])
     assert seen == [0, 1, 2, 3]
     assert np.isfinite(o.last_nute())
+
+
+def test_slab_dumps_are_the_slices(tc, oracle):
+    """DumpRCuts / DumpThCuts / DumpStreamwiseAvg against numpy slices and
+    theta-means of the oracle's own fields (indices and face averages)."""
+    n, nzm, nthm = 9, 6, 10
+    p = params(tc, nthm, nzm, n)
+    o = oracle.Oracle(p)
+    rng = np.random.default_rng(9)
+    f = {}
+    for name in ("qt", "qr", "qz"):
+        a = np.zeros(o.shape, order="F")
+        a[:] = rng.uniform(-1, 1, o.shape)
+        o.set(name, a)
+        f[name] = o.get(name)
+    usrc = 1.0 / o.vec("rc")
+    for k in (1, 4, n):
+        vt, vr, vz = o.dump_rcut(k)
+        assert np.array_equal(vt, f["qt"][k - 1]) and np.array_equal(vz, f["qz"][k - 1])
+        want = 0.5 * (f["qr"][k - 1] * usrc[k - 1] + f["qr"][k] * usrc[k])
+        np.testing.assert_allclose(vr, want, rtol=1e-15, atol=1e-15)
+    vt, vr, vz = o.dump_thcut()
+    assert np.array_equal(vt, f["qt"][:n, :, 0]) and np.array_equal(vz, f["qz"][:n, :, 0])
+    np.testing.assert_allclose(vr, f["qr"][:n, :, 0] * usrc[:n, None], rtol=1e-15)
+    vt, vr, vz = o.dump_streamwise_avg()
+    np.testing.assert_allclose(vt, f["qt"][:n].mean(axis=2), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(vr, (f["qr"][:n] * usrc[:n, None, None]).mean(axis=2),
+                               rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(vz, f["qz"][:n].mean(axis=2), rtol=1e-13, atol=1e-15)

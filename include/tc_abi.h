@@ -208,6 +208,19 @@ int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
 int tc_write_restart(tc_ctx *ctx, const char *path, int ntime);
 int tc_read_restart(tc_ctx *ctx, const char *path, int *ntime);
 
+/* Restart onto another grid (ReadFlowField.F90:63-131, interp.F90:11-303):
+ * interpolates one field (TC_QT, TC_QR or TC_QZ) of a continuation file
+ * written on an old grid into this context.  old is (nro, nzo-1, ntho-1) in
+ * Fortran order, where ntho, nzo, nro are the old nth, nz, nr (nthm+1, ...);
+ * istro, stro are the old grid's ISTR and STR.  As in the reference: the new
+ * resolution may be at most twice the old one in each direction; the radial
+ * grids come from interp.F90's gridnew, which builds ISTR 0 and 6 only; the
+ * result is bilinear in theta and z and takes the lower bracketing radial
+ * plane.  Coarsening by 2 or more in theta or z is rejected (the reference
+ * reads outside its buffer there). */
+int tc_regrid_field(tc_ctx *ctx, int field, const double *old, int ntho, int nzo, int nro,
+                    int istro, double stro);
+
 /* ---------------- statistics ---------------- */
 
 /* The stat_arrays accumulators of StatRoutines.F90 and CalcDissipation.F90,

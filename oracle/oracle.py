@@ -98,6 +98,9 @@ def lib():
         L.or_get_stats.argtypes = [ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int)]
         L.or_last_nute.restype = ctypes.c_double
         L.or_last_nute.argtypes = [ctypes.c_void_p]
+        L.or_regrid.restype = ctypes.c_int
+        L.or_regrid.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_double, _D]
         L.or_dump_rcut.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D]
         L.or_dump_thcut.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.or_dump_streamwise_avg.argtypes = [ctypes.c_void_p, _D, _D, _D]
@@ -231,6 +234,18 @@ class Oracle:
             out[q] = buf[q * self.nrm * self.nzm:(q + 1) * self.nrm * self.nzm].reshape(
                 (self.nrm, self.nzm), order="F")
         return {name: out[q] for q, name in enumerate(STATS)}, n.value
+
+    def regrid(self, name, old, istro, stro):
+        """interp.F90 of one field from an old grid (nro, nzo-1, ntho-1) onto
+        this grid; returns (nr, nzm, nthm)"""
+        old = np.asfortranarray(old, dtype=np.float64)
+        nro, nzom, nthom = old.shape
+        out = np.zeros(self.shape, order="F")
+        rc = lib().or_regrid(self.h, {"qt": 1, "qr": 2, "qz": 3}[name], _dp(old), nthom + 1,
+                             nzom + 1, nro, int(istro), float(stro), _dp(out))
+        if rc:
+            raise ValueError(lib().or_last_error().decode())
+        return out
 
     def dump_rcut(self, kslab):
         """vt, vr, vz at r-plane kslab, each (nzm, nthm)"""

@@ -1785,6 +1785,171 @@ int or_get_stats(or_state *s, double *out, int *nsamples) {
 
 double or_last_nute(or_state *s) { return s->last_nute; }
 
+/* ------------------------------------------------------------------ */
+/* Restart onto another grid: interp.F90:11-303 as ReadFlowField.F90  */
+/* calls it on one rank                                                */
+/* ------------------------------------------------------------------ */
+
+/* gridnew (interp.F90:255-303): 1-based rc[1..n], rm[1..n]; ISTR 0 or 6 */
+static int gridnew(int n, double rint, double rext, int istr, double str, double *rc, double *rm) {
+  if (istr == 0) {
+    for (int i = 1; i <= n; i++) {
+      double x3 = (double)(i - 1) / (double)(n - 1);
+      rc[i] = rext * x3 + rint;
+    }
+  } else if (istr == 6) {
+    const double pi = 2.0 * asin(1.0);
+    const int nclip = (int)str, nrmo = n + nclip + nclip;
+    double *etazm = (double *)calloc((size_t)nrmo + 2, sizeof(double));
+    double *etaz = (double *)calloc((size_t)n + 2, sizeof(double));
+    for (int i = 1; i <= nrmo; i++) etazm[i] = +cos(pi * ((double)i - 0.5) / (double)nrmo);
+    for (int i = 1; i <= n; i++) etaz[i] = etazm[i + nclip];
+    const double delet = etaz[1] - etaz[n];
+    for (int i = 1; i <= n; i++) etaz[i] = etaz[i] / (0.5 * delet);
+    rc[1] = rint;
+    for (int i = 2; i <= n - 1; i++) rc[i] = (rext - rint) * (1. - etaz[i]) * 0.5 + rint;
+    rc[n] = rext;
+    free(etazm);
+    free(etaz);
+  } else {
+    return -1;
+  }
+  for (int i = 1; i <= n - 1; i++) rm[i] = (rc[i] + rc[i + 1]) * 0.5;
+  rm[n] = 2 * rc[n] - rm[n - 1];
+  return 0;
+}
+
+/* interp.F90:53-67: second-order extrapolation to indices 0 and n+1 */
+static void extrap(double *v, int n) {
+  v[0] = 2 * v[1] - v[2];
+  v[n + 1] = 2 * v[n] - v[n - 1];
+}
+
+/* first l in 1..no with old[l] >= x: (l-1, l); none: (no-1, no) */
+static void bracket(const double *old, int no, double x, int *f, int *c) {
+  *f = no - 1;
+  *c = no;
+  for (int l = 1; l <= no; l++)
+    if (old[l] >= x) {
+      *f = l - 1;
+      *c = l;
+      return;
+    }
+}
+
+/* ReadFlowField.F90:63-131 + interp.F90 for one field (intvar 1 qt, 2 qr,
+ * 3 qz).  old: (nro, nzo-1, ntho-1) Fortran order; out: (nr, nzm, nthm).
+ * The reference leaves the halo edges and corners of its old-grid buffer
+ * unset; here they are extrapolated the same way as the faces (radial halo
+ * over the whole padded (j, i) range after the theta and z halos). */
+int or_regrid(or_state *s, int intvar, const double *old, int ntho, int nzo, int nro, int istro,
+              double stro, double *out) {
+  const int nth = s->nth, nz = s->nz, nr = s->nr, nthm = s->nthm, nzm = s->nzm;
+  if (nth > ntho * 2 || nz > nzo * 2 || nr > nro * 2) {
+    snprintf(g_err, sizeof g_err, "new grid more than twice the old resolution");
+    return -1;
+  }
+  const int nthom = ntho - 1, nzom = nzo - 1;
+  const int xs2o = (1 * nzom / nzm) > 1 ? (1 * nzom / nzm) : 1;
+  const int xs3o = (1 * nthom / nthm) > 1 ? (1 * nthom / nthm) : 1;
+  if (xs2o != 1 || xs3o != 1) {
+    snprintf(g_err, sizeof g_err, "coarsening by 2 or more in theta or z is not supported");
+    return -1;
+  }
+  const int xe2o = nzom, xe3o = nthom;
+  const double pi = s->pi, thmax = 2.0 * pi / s->p.lamb;
+  double *tcold = dvec(ntho), *tmold = dvec(ntho), *zcold = dvec(nzo), *zmold = dvec(nzo);
+  double *rcold = dvec(nro), *rmold = dvec(nro);
+  double *tcnew = dvec(nth), *tmnew = dvec(nth), *zcnew = dvec(nz), *zmnew = dvec(nz);
+  double *rcnew = dvec(nr), *rmnew = dvec(nr);
+  int rc = 0;
+  rc |= gridnew(ntho, 0.0, thmax, 0, 1.0, tcold, tmold);
+  rc |= gridnew(nzo, 0.0, s->p.alx3, 0, 1.0, zcold, zmold);
+  rc |= gridnew(nro, s->p.rint, s->p.rext, istro, stro, rcold, rmold);
+  rc |= gridnew(nth, 0.0, thmax, 0, 1.0, tcnew, tmnew);
+  rc |= gridnew(nz, 0.0, s->p.alx3, 0, 1.0, zcnew, zmnew);
+  rc |= gridnew(nr, s->p.rint, s->p.rext, s->p.istr, s->p.str, rcnew, rmnew);
+  double *bufs[] = {tcold, tmold, zcold, zmold, rcold, rmold, tcnew, tmnew, zcnew, zmnew, rcnew, rmnew};
+  if (rc) {
+    snprintf(g_err, sizeof g_err, "interp.F90's gridnew builds ISTR 0 and 6 only");
+    for (size_t q = 0; q < sizeof bufs / sizeof bufs[0]; q++) free(bufs[q]);
+    return -1;
+  }
+  extrap(tcold, ntho); extrap(tmold, ntho); extrap(zcold, nzo); extrap(zmold, nzo);
+  extrap(rcold, nro); extrap(rmold, nro);
+  const double *xold, *yold, *zold, *xnew, *ynew, *znew;
+  switch (intvar) {
+    case 1: xold = tcold; yold = zmold; zold = rmold; xnew = tcnew; ynew = zmnew; znew = rmnew; break;
+    case 2: xold = tmold; yold = zcold; zold = rmold; xnew = tmnew; ynew = zcnew; znew = rmnew; break;
+    default: xold = tmold; yold = zmold; zold = rcold; xnew = tmnew; ynew = zmnew; znew = rcnew; break;
+  }
+  /* arrold(0:nro+1, 0:nzo, 0:ntho) */
+  const int ak = nro + 2, aj = nzo + 1;
+#define AO(k, j, i) a[(size_t)(k) + (size_t)ak * ((size_t)(j) + (size_t)aj * (size_t)(i))]
+  double *a = (double *)calloc((size_t)ak * aj * (ntho + 1), sizeof(double));
+  for (int i = xs3o; i <= xe3o; i++)
+    for (int j = xs2o; j <= xe2o; j++)
+      for (int k = 1; k <= nro; k++)
+        AO(k, j, i) = old[(size_t)(k - 1) + (size_t)nro * ((size_t)(j - 1) + (size_t)nzom * (i - 1))];
+  for (int i = xs3o; i <= xe3o; i++) /* interp.F90:113-118 */
+    for (int j = xs2o; j <= xe2o; j++) {
+      AO(0, j, i) = 2.0 * AO(1, j, i) - AO(2, j, i);
+      AO(nro + 1, j, i) = 2.0 * AO(nro, j, i) - AO(nro - 1, j, i);
+    }
+  for (int j = xs2o; j <= xe2o; j++) /* :120-125 */
+    for (int k = 1; k <= nro; k++) {
+      AO(k, j, xs3o - 1) = 2.0 * AO(k, j, xs3o) - AO(k, j, xs3o + 1);
+      AO(k, j, xe3o + 1) = 2.0 * AO(k, j, xe3o) - AO(k, j, xe3o - 1);
+    }
+  for (int i = xs3o; i <= xe3o; i++) /* :127-132 */
+    for (int k = 1; k <= nro; k++) {
+      AO(k, xs2o - 1, i) = 2.0 * AO(k, xs2o, i) - AO(k, xs2o + 1, i);
+      AO(k, xe2o + 1, i) = 2.0 * AO(k, xe2o, i) - AO(k, xe2o - 1, i);
+    }
+  /* edges and corners the reference leaves unset */
+  for (int k = 1; k <= nro; k++)
+    for (int ii = 0; ii <= 1; ii++) {
+      const int i = ii ? xe3o + 1 : xs3o - 1;
+      AO(k, xs2o - 1, i) = 2.0 * AO(k, xs2o, i) - AO(k, xs2o + 1, i);
+      AO(k, xe2o + 1, i) = 2.0 * AO(k, xe2o, i) - AO(k, xe2o - 1, i);
+    }
+  for (int i = 0; i <= ntho; i++)
+    for (int j = 0; j <= nzo; j++) {
+      if (i >= xs3o && i <= xe3o && j >= xs2o && j <= xe2o) continue;
+      AO(0, j, i) = 2.0 * AO(1, j, i) - AO(2, j, i);
+      AO(nro + 1, j, i) = 2.0 * AO(nro, j, i) - AO(nro - 1, j, i);
+    }
+  for (int i = 1; i <= nthm; i++) /* interp.F90:136-215 */
+    for (int j = 1; j <= nzm; j++)
+      for (int k = 1; k <= nr; k++) {
+        const double bix = xnew[i], biy = ynew[j], biz = znew[k];
+        int bifi, bici, bifj, bicj, bifk, bick;
+        bracket(xold, ntho, bix, &bifi, &bici);
+        bracket(yold, nzo, biy, &bifj, &bicj);
+        bracket(zold, nro, biz, &bifk, &bick);
+        const double bix1 = xold[bifi], bix2 = xold[bici], biy1 = yold[bifj], biy2 = yold[bicj];
+        const double biz1 = zold[bifk], biz2 = zold[bick];
+        /* every corner value is taken on plane bifk (interp.F90:182-190) */
+        const double aficjfk = AO(bifk, bicj, bifi), afifjfk = AO(bifk, bifj, bifi);
+        const double acicjfk = AO(bifk, bicj, bici), acifjfk = AO(bifk, bifj, bici);
+        const double aficjck = aficjfk, afifjck = afifjfk, acicjck = acicjfk, acifjck = acifjfk;
+        const double dxdydz = 1.0 / ((bix2 - bix1) * (biy2 - biy1) * (biz2 - biz1));
+        out[(size_t)(k - 1) + (size_t)nr * ((size_t)(j - 1) + (size_t)nzm * (i - 1))] =
+            (aficjfk * (bix2 - bix) * (biy - biy1) * (biz2 - biz) +
+             afifjfk * (bix2 - bix) * (biy2 - biy) * (biz2 - biz) +
+             acicjfk * (bix - bix1) * (biy - biy1) * (biz2 - biz) +
+             acifjfk * (bix - bix1) * (biy2 - biy) * (biz2 - biz) +
+             aficjck * (bix2 - bix) * (biy - biy1) * (biz - biz1) +
+             afifjck * (bix2 - bix) * (biy2 - biy) * (biz - biz1) +
+             acicjck * (bix - bix1) * (biy - biy1) * (biz - biz1) +
+             acifjck * (bix - bix1) * (biy2 - biy) * (biz - biz1)) * dxdydz;
+      }
+#undef AO
+  free(a);
+  for (size_t q = 0; q < sizeof bufs / sizeof bufs[0]; q++) free(bufs[q]);
+  return 0;
+}
+
 /* DumpRCuts.F90:28-36; outputs (nzm, nthm), Fortran order */
 void or_dump_rcut(or_state *s, int kslab, double *vt, double *vr, double *vz) {
   const int nzm = s->nzm;

@@ -97,6 +97,11 @@ void or_calc_stats(or_state *s);          /* CalcStats incl. CalcVorticity */
 double or_calc_dissipation(or_state *s);  /* CalcDissipation; returns nute */
 int or_get_stats(or_state *s, double *out, int *nsamples);
 double or_last_nute(or_state *s);         /* NaN before the first call */
+/* restart onto another grid (ReadFlowField.F90:63-131, interp.F90);
+ * intvar 1 qt, 2 qr, 3 qz; old (nro, nzo-1, ntho-1), out (nr, nzm, nthm);
+ * ntho, nzo, nro are the old nth, nz, nr. */
+int or_regrid(or_state *s, int intvar, const double *old, int ntho, int nzo, int nro, int istro,
+              double stro, double *out);
 /* slab dumps (DumpRCuts, DumpThCuts, DumpStreamwiseAvg) */
 void or_dump_rcut(or_state *s, int kslab, double *vt, double *vr, double *vz);
 void or_dump_thcut(or_state *s, double *vt, double *vr, double *vz);

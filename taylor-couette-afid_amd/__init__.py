@@ -63,6 +63,7 @@ ABI_SYMBOLS = [
     "tc_write_restart", "tc_read_restart",
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
+    "tc_regrid_field",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -169,6 +170,8 @@ def lib():
     L.tc_set_stats.argtypes = [_V, _D, ctypes.c_int]
     L.tc_last_dissipation.argtypes = [_V, _D]
     L.tc_dump_rcut.argtypes = [_V, ctypes.c_int, _D, _D, _D]
+    L.tc_regrid_field.argtypes = [_V, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_double]
     L.tc_dump_thcut.argtypes = [_V, _D, _D, _D]
     L.tc_dump_streamwise_avg.argtypes = [_V, _D, _D, _D]
     _LIB = L
@@ -389,6 +392,14 @@ class Solver:
         for q, name in enumerate(STATS):
             buf[q * m:(q + 1) * m] = np.asarray(stats[name], dtype=np.float64).reshape(-1, order="F")
         _check(lib().tc_set_stats(self.h, _dp(buf), int(nsamples)))
+
+    def regrid(self, name, old, istro, stro):
+        """ReadFlowField's interpolation (interp.F90) of one field from an old
+        grid, old shaped (nro, nzo-1, ntho-1), into this context."""
+        old = np.asfortranarray(old, dtype=np.float64)
+        nro, nzom, nthom = old.shape
+        _check(lib().tc_regrid_field(self.h, FIELDS[name], _dp(old), nthom + 1, nzom + 1, nro,
+                                     int(istro), float(stro)))
 
     # --- slab dumps (DumpRCuts.F90, DumpThCuts.F90, DumpStreamwiseAvg.F90) ---
     def DumpRCut(self, kslab):

@@ -19,7 +19,7 @@ HOST = os.path.join(HERE, "tc_host")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("TC_FC", "/opt/rocm/bin/amdflang")
 SOURCES = ["tc_grid.cpp", "tc_abi.cpp", "tc_batch.cpp", "tc_kernels.hip", "tc_poisson.hip",
-           "tc_stats.hip"]
+           "tc_stats.hip", "tc_regrid.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     # no packed-FP32 instructions anywhere (all arithmetic here is FP64)

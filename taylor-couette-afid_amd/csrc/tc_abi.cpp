@@ -749,6 +749,12 @@ int tc_read_restart(tc_ctx *c, const char *path, int *ntime) {
   return 0;
 }
 
+int tc_regrid_field(tc_ctx *c, int field, const double *old, int ntho, int nzo, int nro, int istro,
+                    double stro) {
+  CTXCHK(c);
+  return tc::regrid_field(c, field, old, ntho, nzo, nro, istro, stro);
+}
+
 int tc_destroy(tc_ctx *c) {
   destroy_ctx(c);
   return 0;

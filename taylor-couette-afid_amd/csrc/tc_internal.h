@@ -192,6 +192,8 @@ void launch_velocity_stats(tc_ctx *c);
 void launch_diagnostics(tc_ctx *c);
 void launch_stats(tc_ctx *c, bool stats, bool diss);
 void launch_dump(tc_ctx *c, int kind, int k);
+int regrid_field(tc_ctx *c, int field, const double *old, int ntho, int nzo, int nro, int istro,
+                 double stro);
 void launch_copy(tc_ctx *c, int dst, int src);
 
 // grouped launches over n instances of one shape (c: any instance of the

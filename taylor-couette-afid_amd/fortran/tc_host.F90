@@ -14,7 +14,8 @@
 !                instead of the routine-by-routine calls
 !   --ic ...     seeded perturbed Couette start, projected divergence free;
 !                without it NREAD=0 uses CreateInitialConditions and
-!                NREAD=1 reads continua_{qt,qr,qz}.bin
+!                NREAD=1 reads continua_{qt,qr,qz}.bin with
+!                continua_master.txt, interpolating from another grid
 !   --restart F  continue from a tc_write_restart checkpoint (bit for bit)
 !   --checkpoint F  write a checkpoint of the final state
 ! Outputs: torquenu.out (time, inner, outer; e14.6 as CalcTorque.F90:59),
@@ -193,6 +194,14 @@ module tc_abi
       real(c_double), intent(out) :: a(*)
       integer(c_int), intent(out) :: nsamples
     end function
+    integer(c_int) function tc_regrid_field(ctx, f, a, ntho, nzo, nro, istro, stro) &
+        bind(C, name='tc_regrid_field')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: f, ntho, nzo, nro, istro
+      real(c_double), intent(in) :: a(*)
+      real(c_double), value :: stro
+    end function
     integer(c_int) function tc_dump_rcut(ctx, k, vt, vr, vz) bind(C, name='tc_dump_rcut')
       import
       type(c_ptr), value :: ctx
@@ -364,7 +373,11 @@ subroutine PerturbedCouetteStart
   call tc_check(tc_project(ctx), 'projection')
 end subroutine PerturbedCouetteStart
 
-! raw-binary restart: continua_<field>.bin holding (nr,nzm,nthm) FP64
+! ReadFlowField.F90:12-152 with raw files: continua_master.txt holds the
+! scalars of continua_master.h5 (nr nth nz time rint rext istr str alx3
+! lamb, as WriteFlowField writes them) and continua_<field>.bin the
+! (nr, nzm, nthm) FP64 fields of the old grid.  Another grid or stretching
+! is interpolated on the device (tc_regrid_field, interp.F90).
 subroutine ReadFlowField
   use param
   use tc_abi
@@ -372,14 +385,41 @@ subroutine ReadFlowField
   real, allocatable :: a(:,:,:)
   character(len=2), parameter :: names(3) = ['qt', 'qr', 'qz']
   integer(c_int), parameter :: ids(3) = [TC_QT, TC_QR, TC_QZ]
-  integer :: q, u
-  allocate(a(nr, nzm, nthm))
-  do q = 1, 3
-    open(newunit=u, file='continua_'//names(q)//'.bin', access='stream', form='unformatted', status='old')
-    read(u) a
-    close(u)
-    call tc_check(tc_upload_field(ctx, ids(q), a, TC_LAYOUT_FORTRAN), 'upload '//names(q))
-  end do
+  integer :: q, u, ios, nro, ntho, nzo, istro
+  real :: timeo, rinto, rexto, stro, alx3o, lambo
+  open(newunit=u, file='continua_master.txt', status='old', iostat=ios)
+  if (ios /= 0) then
+    write(6, *) 'Continuation files not found'
+    stop 3
+  end if
+  read(u, *) nro, ntho, nzo, timeo, rinto, rexto, istro, stro, alx3o, lambo
+  close(u)
+  if (nzo /= nzm + 1 .or. nro /= nr .or. ntho /= nthm + 1 .or. istro /= bou%istr .or. &
+      stro /= bou%str) then
+    write(6, *) 'Interpolating new grid'
+    allocate(a(nro, nzo - 1, ntho - 1))
+    do q = 1, 3
+      open(newunit=u, file='continua_'//names(q)//'.bin', access='stream', form='unformatted', &
+           status='old')
+      read(u) a
+      close(u)
+      call tc_check(tc_regrid_field(ctx, ids(q), a, int(ntho, c_int), int(nzo, c_int), &
+                                    int(nro, c_int), int(istro, c_int), stro), 'regrid '//names(q))
+    end do
+  else
+    allocate(a(nr, nzm, nthm))
+    do q = 1, 3
+      open(newunit=u, file='continua_'//names(q)//'.bin', access='stream', form='unformatted', &
+           status='old')
+      read(u) a
+      close(u)
+      call tc_check(tc_upload_field(ctx, ids(q), a, TC_LAYOUT_FORTRAN), 'upload '//names(q))
+    end do
+  end if
+  time = timeo
+  if (bou%ireset /= 0) time = 0.d0  ! resetlogstime
+  tmax = tmax + time
+  bou%tmax = tmax
 end subroutine ReadFlowField
 
 subroutine WriteFlowField
@@ -397,6 +437,10 @@ subroutine WriteFlowField
     write(u) a
     close(u)
   end do
+  ! the scalars of continua_master.h5, WriteFlowField.F90:24-49
+  open(newunit=u, file='continua_master.txt', status='replace')
+  write(u, *) nr, nthm + 1, nzm + 1, time, bou%rint, bou%rext, bou%istr, bou%str, bou%alx3, bou%lamb
+  close(u)
 end subroutine WriteFlowField
 
 ! ---- the hot-path routines: one device call each ----

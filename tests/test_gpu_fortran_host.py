@@ -205,3 +205,50 @@ def test_fortran_host_slab_dumps(tc, oracle, tmp_path):
     for n in ("qt", "qr", "qz"):
         got = np.fromfile(tmp_path / "fullff" / f"{n}_{stamp}.bin").reshape(shape, order="F")
         assert np.array_equal(got, f[n]), n
+
+
+def test_fortran_host_restart_onto_a_finer_grid(tc, oracle, tmp_path):
+    """NREAD=1 with continua_master.txt from a coarser run: the host
+    interpolates on the device (ReadFlowField + interp.F90), continues the
+    time of the old run, and the run equals the same restart driven from
+    Python through the C-ABI."""
+    if not os.path.exists(HOST):
+        pytest.fail("tc_host is not built")
+    base = tc.configs.scaled(tc.configs.small_64(), 16, 12, 17)
+    base.update(ntst=1, tout=1e-9, sizepert=0.1, omegapert=0.5, idtv=1, dtmin=1e-3, dtmax=5e-3,
+                limitcfl=0.5, virotext=-0.2, istr=6, str=2.0)
+    a = tmp_path / "coarse"
+    a.mkdir()
+    _run_host(a, base, fused=True)
+    master = open(a / "continua_master.txt").read().split()
+    t_old = float(master[3])
+    fine = dict(base, nthm=32, nzm=24, nrm=32, nread=1, ireset=0, ntst=3)
+    b = tmp_path / "fine"
+    b.mkdir()
+    for f in ("qt", "qr", "qz"):
+        (b / f"continua_{f}.bin").write_bytes((a / f"continua_{f}.bin").read_bytes())
+    (b / "continua_master.txt").write_text(open(a / "continua_master.txt").read())
+    write_bou(b / "bou.in", fine)
+    r = subprocess.run([HOST, "--fused"], cwd=b, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "exit code 555" in r.stdout, r.stdout + r.stderr
+    assert "Interpolating new grid" in r.stdout
+    tq = np.loadtxt(b / "torquenu_full.out", ndmin=2)
+    assert tq[0, 0] > t_old
+    # the same restart through the Python mirror
+    shape_old = (base["nrm"] + 1, base["nzm"], base["nthm"])
+    dev = tc.Solver(fine)
+    for f in ("qt", "qr", "qz"):
+        old = np.fromfile(a / f"continua_{f}.bin").reshape(shape_old, order="F")
+        dev.regrid(f, old, base["istr"], base["str"])
+    dev.set_time(t_old, 0.0)
+    rows = []
+    for n in range(fine["ntst"] + 1):
+        code, d = dev.step(n)
+        rows.append((d["time"], d["torque_in"], d["torque_out"]))
+    assert code == 555
+    np.testing.assert_allclose(tq, np.array(rows), rtol=1e-10)
+    shape = (fine["nrm"] + 1, fine["nzm"], fine["nthm"])
+    for f in ("qt", "qr", "qz"):
+        got = np.fromfile(b / f"continua_{f}.bin").reshape(shape, order="F")
+        want = dev.get(f)
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max(), f

@@ -124,10 +124,15 @@ def gpu_leg(tc, p, seed, amp, steps, warmup):
     for _ in range(5):
         dev.copy_field("rhs", "dph")
     ms, cnt = dev.kernel_time("copy", reset=True)
-    dev.kernel_timing(False)
     copy_gbs = 2 * 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"] / (ms / cnt / 1e3) / 1e9
+    # STREAM triad (two reads, one write) over three whole fields
+    for _ in range(5):
+        dev.stream_triad("rhs", "dph", "dq")
+    ms, cnt = dev.kernel_time("triad", reset=True)
+    triad_gbs = 3 * 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"] / (ms / cnt / 1e3) / 1e9
+    dev.kernel_timing(False)
     dev.close()
-    return (t1 - t0), times, last, copy_gbs
+    return (t1 - t0), times, last, copy_gbs, triad_gbs
 
 
 def cpu_leg(p, seed, amp, max_seconds):
@@ -155,6 +160,28 @@ def cpu_leg(p, seed, amp, max_seconds):
             break
     orc.close()
     return n, t_steps, threads
+
+
+def host_info():
+    """CPU model, sockets and logical CPUs from /proc/cpuinfo, and the
+    host's STREAM triad bandwidth on the OpenMP threads."""
+    model, sockets, ncpu = "unknown", set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name":
+                    model = v.strip()
+                    ncpu += 1
+                elif k == "physical id":
+                    sockets.add(v.strip())
+    except OSError:
+        pass
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return {"model": model, "sockets": max(len(sockets), 1), "logical_cpus": ncpu,
+            "triad_gbs": round(O.stream_triad_gbs(1 << 26, 5), 1)}
 
 
 def _alone_step_ms(tc, p, fields, steps=8, warmup=2, tuning=None):
@@ -410,7 +437,7 @@ def main():
     seed, amp = tc.configs.SEEDS[a.workload], tc.configs.AMPLITUDE[a.workload]
     npts = p["nthm"] * p["nzm"] * p["nrm"]
 
-    elapsed, ktimes, last, copy_gbs = gpu_leg(tc, p, seed, amp, a.steps, a.warmup)
+    elapsed, ktimes, last, copy_gbs, triad_gbs = gpu_leg(tc, p, seed, amp, a.steps, a.warmup)
     ms_step = 1e3 * elapsed / a.steps
     value = npts * a.steps / elapsed / 1e6
 
@@ -453,7 +480,8 @@ def main():
                      "traffic": traffic_of(a.workload, dom),
                      "avg_launch_ms": round(dom_ms / dom_cnt, 4),
                      "bytes_per_launch": kb[dom],
-                     "copy_kernel_gbs": round(copy_gbs, 1)},
+                     "copy_kernel_gbs": round(copy_gbs, 1),
+                     "triad_kernel_gbs": round(triad_gbs, 1)},
     }
     if a.sweep_steps > 0:
         out["sweep"] = sweep_leg(tc, a.sweep_steps)
@@ -467,7 +495,8 @@ def main():
                                "cores": threads, "kind": "port",
                                "sample": f"{n} RK3 step(s) of the same {p['nthm']}x{p['nzm']}x"
                                          f"{p['nrm']} workload from the same start, oracle "
-                                         f"(C restatement, OpenMP), {t:.1f} s"}
+                                         f"(C restatement, OpenMP), {t:.1f} s",
+                               "host": host_info()}
     print(json.dumps(out))
 
 

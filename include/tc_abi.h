@@ -191,6 +191,10 @@ int tc_kernel_timing_read(tc_ctx *ctx, const char *kernel, double *total_ms,
  * hardware byte counters into bytes per launch. */
 int tc_copy_field(tc_ctx *ctx, int dst, int src);
 
+/* STREAM triad a = b + s*c over three whole fields: the achievable
+ * two-read-one-write bandwidth of the box, reported beside the copy. */
+int tc_stream_triad(tc_ctx *ctx, int a, int b, int c, double s);
+
 /* Device bytes held by the context (fields + scratch). */
 int tc_device_bytes(tc_ctx *ctx, unsigned long long *bytes);
 

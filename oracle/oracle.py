@@ -104,6 +104,8 @@ def lib():
         L.or_dump_rcut.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D]
         L.or_dump_thcut.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.or_dump_streamwise_avg.argtypes = [ctypes.c_void_p, _D, _D, _D]
+        L.or_stream_triad.restype = ctypes.c_double
+        L.or_stream_triad.argtypes = [ctypes.c_longlong, ctypes.c_int]
         L.or_step.restype = ctypes.c_int
         L.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]
         for n in ["or_dgttrf", "or_zgttrf"]:
@@ -268,6 +270,11 @@ class Oracle:
         code = lib().or_step(self.h, ntime, walltime, _dp(o))
         return code, dict(zip(["cfl", "dt", "time", "dmax", "torque_in", "torque_out",
                                "vmax1", "vmax2", "vmax3"], o))
+
+
+def stream_triad_gbs(n=1 << 26, reps=5):
+    """host STREAM triad bandwidth (GB/s) on the OpenMP threads"""
+    return lib().or_stream_triad(int(n), int(reps))
 
 
 # ---- LAPACK / FFT restatements, numpy in and out ----

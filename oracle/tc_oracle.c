@@ -1995,6 +1995,40 @@ void or_dump_streamwise_avg(or_state *s, double *vt, double *vr, double *vz) {
       }
 }
 
+/* STREAM triad a = b + s*c on the host cores (OpenMP): the host's
+ * achievable bandwidth, reported beside the CPU baseline.  Returns GB/s of
+ * the best of reps passes over n doubles per array. */
+double or_stream_triad(long long n, int reps) {
+  double *a = (double *)malloc(sizeof(double) * (size_t)n);
+  double *b = (double *)malloc(sizeof(double) * (size_t)n);
+  double *c = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!a || !b || !c) {
+    free(a); free(b); free(c);
+    return 0.0;
+  }
+#pragma omp parallel for schedule(static)
+  for (long long q = 0; q < n; q++) {
+    a[q] = 0.0;
+    b[q] = 1.0;
+    c[q] = 2.0;
+  }
+  double best = 1e30;
+  for (int r = 0; r < reps; r++) {
+#ifdef _OPENMP
+    const double t0 = omp_get_wtime();
+#endif
+#pragma omp parallel for schedule(static)
+    for (long long q = 0; q < n; q++) a[q] = b[q] + 3.0 * c[q];
+#ifdef _OPENMP
+    const double t = omp_get_wtime() - t0;
+    if (t < best) best = t;
+#endif
+  }
+  const double sum = a[n / 2];
+  free(a); free(b); free(c);
+  return sum > 0.0 && best < 1e30 ? 24.0 * (double)n / best / 1e9 : 0.0;
+}
+
 /* ------------------------------------------------------------------ */
 /* main.F90:152-223: one iteration of the step loop                    */
 /* ------------------------------------------------------------------ */

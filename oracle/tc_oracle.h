@@ -117,6 +117,9 @@ void or_project(or_state *s);
  * vmax1..3. */
 int or_step(or_state *s, int ntime, double walltime, double *outs9);
 
+/* host STREAM triad bandwidth, GB/s (benchmark context only) */
+double or_stream_triad(long long n, int reps);
+
 /* LAPACK restatements (partial pivoting), 0-based arrays. */
 int or_dgttrf(int n, double *dl, double *d, double *du, double *du2, int *ipiv);
 void or_dgttrs(int n, const double *dl, const double *d, const double *du,

@@ -63,7 +63,7 @@ ABI_SYMBOLS = [
     "tc_write_restart", "tc_read_restart",
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
-    "tc_regrid_field",
+    "tc_regrid_field", "tc_stream_triad",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -149,6 +149,7 @@ def lib():
                                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
     L.tc_device_bytes.argtypes = [_V, ctypes.POINTER(ctypes.c_ulonglong)]
     L.tc_copy_field.argtypes = [_V, ctypes.c_int, ctypes.c_int]
+    L.tc_stream_triad.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     _T = ctypes.POINTER(Tuning)
     L.tc_create_tuned.argtypes = [ctypes.POINTER(Params), _T, ctypes.POINTER(_V)]
     L.tc_set_tuning.argtypes = [_V, _T]
@@ -437,6 +438,9 @@ class Solver:
 
     def copy_field(self, dst, src):
         _check(lib().tc_copy_field(self.h, FIELDS[dst], FIELDS[src]))
+
+    def stream_triad(self, a, b, c, s=3.0):
+        _check(lib().tc_stream_triad(self.h, FIELDS[a], FIELDS[b], FIELDS[c], float(s)))
 
     def device_bytes(self):
         v = ctypes.c_ulonglong()

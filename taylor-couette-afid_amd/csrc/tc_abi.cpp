@@ -1206,6 +1206,15 @@ int tc_copy_field(tc_ctx *c, int dst, int src) {
   return launch_status();
 }
 
+int tc_stream_triad(tc_ctx *c, int a, int b, int cc, double s) {
+  CTXCHK(c);
+  if (check_field(c, a) || check_field(c, b) || check_field(c, cc)) return -1;
+  if (a == b || a == cc || b == cc) return fail("tc_stream_triad needs three different fields");
+  c->cfl_valid = false;
+  tc::launch_triad(c, a, b, cc, s);
+  return launch_status();
+}
+
 int tc_device_bytes(tc_ctx *c, unsigned long long *bytes) {
   CTXCHK(c);
   *bytes = c->device_bytes;

@@ -195,6 +195,7 @@ void launch_dump(tc_ctx *c, int kind, int k);
 int regrid_field(tc_ctx *c, int field, const double *old, int ntho, int nzo, int nro, int istro,
                  double stro);
 void launch_copy(tc_ctx *c, int dst, int src);
+void launch_triad(tc_ctx *c, int a, int b, int cc, double s);
 
 // grouped launches over n instances of one shape (c: any instance of the
 // group, for the shape and the tuning); tab: device table of n StepDesc of

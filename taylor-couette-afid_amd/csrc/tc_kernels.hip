@@ -1318,6 +1318,29 @@ void launch_copy(tc_ctx *c, int dst, int src) {
   timer_end(c);
 }
 
+// STREAM triad a = b + s*c over whole fields, two doubles per thread.
+__global__ __launch_bounds__(256) void k_triad(double2 *__restrict__ a, const double2 *__restrict__ b,
+                                               const double2 *__restrict__ cc, double s, long long n2,
+                                               double *__restrict__ a1, const double *__restrict__ b1,
+                                               const double *__restrict__ c1, long long n) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n2) {
+    const double2 x = b[q], y = cc[q];
+    a[q] = make_double2(x.x + s * y.x, x.y + s * y.y);
+  }
+  if (q == 0 && (n & 1)) a1[n - 1] = b1[n - 1] + s * c1[n - 1];
+}
+
+void launch_triad(tc_ctx *c, int a, int b, int cc, double s) {
+  timer_begin(c, "triad");
+  const long long n2 = c->N / 2;
+  const unsigned blocks = (unsigned)((n2 + 255) / 256 > 0 ? (n2 + 255) / 256 : 1);
+  hipLaunchKernelGGL(k_triad, dim3(blocks), dim3(256), 0, c->stream, (double2 *)c->f[a],
+                     (const double2 *)c->f[b], (const double2 *)c->f[cc], s, n2, c->f[a], c->f[b],
+                     c->f[cc], c->N);
+  timer_end(c);
+}
+
 // ---------------------------------------------------------------------------
 // Grouped launches over the instances of one shape group
 // ---------------------------------------------------------------------------

@@ -648,7 +648,8 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
 template <int PQ>
 __global__ __launch_bounds__(512) void k_poisson_tridiag(GridDev g, double2 *__restrict__ spec,
                                                          double inv_norm) {
-  poisson_body<PQ>(blockIdx.x, g, spec, inv_norm);
+  // consecutive line groups on one XCD (a permutation: block 0 stays 0)
+  poisson_body<PQ>(xcd_tile(blockIdx.x, gridDim.x), g, spec, inv_norm);
 }
 
 // Grouped: blockIdx.y = instance.

@@ -276,10 +276,21 @@ DEVI void theta_fwd_body(const long long pb, const double *__restrict__ in, cons
               (w.qzp[e] - w.qz[e]) * g.dxz) *
              dv.usdt;
     };
-    auto ld = [&](int e) { return make_double2(v0 ? div(w0, e) : 0.0, v1 ? div(w1, e) : 0.0); };
+    // rows past nrows read row 0 (w clamped) and are zeroed after, so the
+    // loads carry no condition
+    auto ld = [&](int e) {
+      const double d0 = div(w0, e), d1 = div(w1, e);
+      return make_double2(v0 ? d0 : 0.0, v1 ? d1 : 0.0);
+    };
     auto st = [&](int, double2) {};
     if constexpr (STAGE) {
-      for (int e = t; e < n; e += T) buf[pidx(off + e, sh)] = ld(e);
+      // n <= ME * T: ME rounds, the last ones clamped to element n-1
+#pragma unroll
+      for (int q = 0; q < ME; q++) {
+        const int e = t + q * T;
+        const double2 v = ld(e < n ? e : n - 1);
+        if (e < n) buf[pidx(off + e, sh)] = v;
+      }
       __syncthreads();
       fft_line<ME, false, false>(buf, off, 1, pl, t, T, tw, -1.0, ld, st);
     } else {

@@ -923,6 +923,70 @@ __global__ __launch_bounds__(256) void k_correct_both(GridDev g, double *__restr
   if (kc == g.nrm) qr[IX(g, g.nr, j, i)] = 0.0;
 }
 
+// CorrectVelocity + CorrectPressure + the qr wall reset on a theta pair
+// (nthm even): one lane owns points i, i+1 (i even) and moves them as 16-byte
+// loads and stores; the arithmetic per point is that of correct_velocity_at
+// and correct_pressure_at.
+DEVI void correct_pair(const GridDev &g, double *__restrict__ qt, double *__restrict__ qr,
+                       double *__restrict__ qz, double *__restrict__ pr,
+                       const double *__restrict__ dph, int kc, int j, int i, double dt, double al,
+                       double be) {
+  // no FMA contraction: every caller (plain and grouped) rounds the same way
+#pragma clang fp contract(off)
+  const int im = wrap_m(i, g.nthm), ip = wrap_p(i + 1, g.nthm);
+  const int jp = wrap_p(j, g.nzm), jm = wrap_m(j, g.nzm);
+  const int kp = kc < g.nrm ? kc + 1 : g.nrm, km = kc > 1 ? kc - 1 : 1;
+  const long long o = IX(g, kc, j, i);
+  auto ld2 = [](const double *a, long long x) { return *reinterpret_cast<const double2 *>(a + x); };
+  auto st2 = [](double *a, long long x, double u, double v) {
+    *reinterpret_cast<double2 *>(a + x) = make_double2(u, v);
+  };
+  const double2 d = ld2(dph, o), djm = ld2(dph, IX(g, kc, jm, i)), djp = ld2(dph, IX(g, kc, jp, i));
+  const double2 dkm = ld2(dph, IX(g, km, j, i)), dkp = ld2(dph, IX(g, kp, j, i));
+  const double dim_ = dph[IX(g, kc, j, im)], dip = dph[IX(g, kc, j, ip)];
+  const double2 t = ld2(qt, o), r = ld2(qr, o), z = ld2(qz, o), p = ld2(pr, o);
+  // CorrectVelocity.F90:32-40
+  st2(qt, o, t.x - (d.x - dim_) * dt * al * g.dxt * g.usrm[kc],
+      t.y - (d.y - d.x) * dt * al * g.dxt * g.usrm[kc]);
+  st2(qr, o, r.x - (d.x - dkm.x) * dt * al * g.rudx2c[kc], r.y - (d.y - dkm.y) * dt * al * g.rudx2c[kc]);
+  st2(qz, o, z.x - (d.x - djm.x) * al * dt * g.dxz, z.y - (d.y - djm.y) * al * dt * g.dxz);
+  // CorrectPressure.F90:37-46 (amm = apph, be = beta)
+  const double amm = g.apph[kc], acc = g.acph[kc], app = g.apph[kc];
+  const double px = p.x + d.x -
+                    be * ((d.y - 2.0 * d.x + dim_) * g.udx1qm[kc] + (djp.x - 2.0 * d.x + djm.x) * g.dxzq +
+                          (dkp.x * app + d.x * acc + dkm.x * amm));
+  const double py = p.y + d.y -
+                    be * ((dip - 2.0 * d.y + d.x) * g.udx1qm[kc] + (djp.y - 2.0 * d.y + djm.y) * g.dxzq +
+                          (dkp.y * app + d.y * acc + dkm.y * amm));
+  st2(pr, o, px, py);
+  // CorrectVelocity.F90:46-47
+  if (kc == 1) st2(qr, IX(g, 1, j, i), 0.0, 0.0);
+  if (kc == g.nrm) st2(qr, IX(g, g.nr, j, i), 0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_correct_both2(GridDev g, double *__restrict__ qt,
+                                                       double *__restrict__ qr,
+                                                       double *__restrict__ qz,
+                                                       double *__restrict__ pr,
+                                                       const double *__restrict__ dph, double dt,
+                                                       double al, double be) {
+  const int i = 2 * (blockIdx.x * 64 + threadIdx.x);
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nthm || j >= g.nzm) return;
+  correct_pair(g, qt, qr, qz, pr, dph, blockIdx.z + 1, j, i, dt, al, be);
+}
+
+// Grouped, theta pairs: blockIdx.z = instance * nrm + plane.
+__global__ __launch_bounds__(256) void k_correct_both2_g(const StepDesc *__restrict__ tab, int nrm) {
+  const int inst = blockIdx.z / nrm, kc = blockIdx.z - inst * nrm + 1;
+  const StepDesc &d = tab[inst];
+  const int i = 2 * (blockIdx.x * 64 + threadIdx.x);
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.g.nthm || j >= d.g.nzm) return;
+  correct_pair(d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.f[TC_PR], d.f[TC_DPH], kc, j, i, d.dt, d.al,
+               d.beta);
+}
+
 // Grouped: blockIdx.z = instance * nrm + plane.
 __global__ __launch_bounds__(256) void k_correct_both_g(const StepDesc *__restrict__ tab, int nrm) {
   const int inst = blockIdx.z / nrm, kc = blockIdx.z - inst * nrm + 1;
@@ -961,9 +1025,16 @@ void launch_correct_pressure(tc_ctx *c) {
 }
 void launch_correct_both(tc_ctx *c) {
   timer_begin(c, "correct_both");
-  hipLaunchKernelGGL(k_correct_both, point_grid(c, c->nrm), dim3(64, 4), 0, c->stream, c->gd,
-                     c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR], c->f[TC_DPH], c->dt,
-                     c->al, c->beta);
+  if (c->nthm % 2 == 0)
+    hipLaunchKernelGGL(k_correct_both2,
+                       dim3((unsigned)((c->nthm / 2 + 63) / 64), (unsigned)((c->nzm + 3) / 4),
+                            (unsigned)c->nrm),
+                       dim3(64, 4), 0, c->stream, c->gd, c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ],
+                       c->f[TC_PR], c->f[TC_DPH], c->dt, c->al, c->beta);
+  else
+    hipLaunchKernelGGL(k_correct_both, point_grid(c, c->nrm), dim3(64, 4), 0, c->stream, c->gd,
+                       c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR], c->f[TC_DPH], c->dt,
+                       c->al, c->beta);
   timer_end(c);
 }
 void launch_zero_wall_qr(tc_ctx *c) {
@@ -1366,9 +1437,15 @@ void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dy
 }
 
 void launch_correct_both_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t st) {
-  const dim3 grid((unsigned)((c->nthm + 63) / 64), (unsigned)((c->nzm + 3) / 4),
-                  (unsigned)(c->nrm * n));
-  hipLaunchKernelGGL(k_correct_both_g, grid, dim3(64, 4), 0, st, tab, c->nrm);
+  if (c->nthm % 2 == 0) {
+    const dim3 grid((unsigned)((c->nthm / 2 + 63) / 64), (unsigned)((c->nzm + 3) / 4),
+                    (unsigned)(c->nrm * n));
+    hipLaunchKernelGGL(k_correct_both2_g, grid, dim3(64, 4), 0, st, tab, c->nrm);
+  } else {
+    const dim3 grid((unsigned)((c->nthm + 63) / 64), (unsigned)((c->nzm + 3) / 4),
+                    (unsigned)(c->nrm * n));
+    hipLaunchKernelGGL(k_correct_both_g, grid, dim3(64, 4), 0, st, tab, c->nrm);
+  }
 }
 
 void launch_cfl_check_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, BatchRes *res, int n,

@@ -474,14 +474,17 @@ __global__ __launch_bounds__(NT) void k_fft_z_g(const StepDesc *__restrict__ tab
 template <int PQ>  // max rows per thread
 DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restrict__ spec,
                        double inv_norm) {
-  extern __shared__ double red[];  // [LB][2*PT][4]: A, C, D.x, D.y
+  // [LB][2*PT][4]: A, C, D.x, D.y, each line padded by 4 doubles so that the
+  // lines' rows start 8 banks apart (poisson_lds_doubles)
+  extern __shared__ double red[];
   const int LB = blockDim.x, PT = blockDim.y;
   const int lx = threadIdx.x, pp = threadIdx.y;
   const int tid = pp * LB + lx, nthr = LB * PT;
   const long long M = (long long)g.nzm * g.sstr;
   const long long m = blk * LB + lx;
   const int n = g.nrm;
-  double2 *gline = (double2 *)(red + (size_t)LB * 2 * PT * 4);  // gauge line, n values
+  const int LS = 2 * PT * 4 + 4;                                // padded line stride
+  double2 *gline = (double2 *)(red + (size_t)LB * LS);            // gauge line, n values
   double *gcp = (double *)(gline + n);                          // its multipliers
   double *cam = gcp + n, *cap = cam + n, *cac = cap + n, *cus = cac + n;  // row k at [k-1]
   const bool gblock = (blk == 0);
@@ -572,7 +575,7 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
 #pragma unroll
   for (int i = 1; i < PQ; i++)
     if (i == S - 1) { aL = ar[i]; cL = cr[i]; dL = d[i]; }
-  double *rl = red + (size_t)lx * (2 * PT) * 4;
+  double *rl = red + (size_t)lx * LS;
   rl[(2 * pp) * 4 + 0] = A0;
   rl[(2 * pp) * 4 + 1] = C0;
   rl[(2 * pp) * 4 + 2] = D0.x;
@@ -673,6 +676,12 @@ __global__ __launch_bounds__(512) void k_poisson_tridiag_g(const StepDesc *__res
 // Launch shape of k_poisson_tridiag for a line of n rows: PT partitions of at
 // most PQ rows (and at least 2), LB lines per block with LB*PT <= 512
 // (<= 256 for PQ = 13, so that three blocks share a CU).
+// Dynamic LDS of k_poisson_tridiag: the padded reduced systems of lb lines
+// plus the gauge line (2n), its multipliers (n) and four coefficient tables.
+size_t poisson_lds_doubles(int pt, int lb, int n) {
+  return (size_t)lb * (2 * (size_t)pt * 4 + 4) + 7 * (size_t)n;
+}
+
 void poisson_part_shape(int n, int PQ, int &pt, int &lb) {
   pt = (n + PQ - 1) / PQ;
   if (pt < 1) pt = 1;
@@ -811,7 +820,7 @@ void launch_poisson(tc_ctx *c, bool with_divergence, double usdt) {
     int pt, lb;
     const int pq = c->poisson_pq;
     poisson_part_shape(c->nrm, pq, pt, lb);
-    const size_t lds = sizeof(double) * (4 * 2 * (size_t)pt * lb + 7 * (size_t)c->nrm);
+    const size_t lds = sizeof(double) * poisson_lds_doubles(pt, lb, c->nrm);
     const dim3 grid((unsigned)((M + lb - 1) / lb)), block(lb, pt);
     const double inv_norm = 1.0 / ((double)c->nthm * (double)c->nzm);
     if (pq == 13)
@@ -849,7 +858,7 @@ void launch_poisson_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t st) {
     if (pass == 1) {
       int ptn, lb;
       poisson_part_shape(c->nrm, c->poisson_pq, ptn, lb);
-      const size_t lds = sizeof(double) * (4 * 2 * (size_t)ptn * lb + 7 * (size_t)c->nrm);
+      const size_t lds = sizeof(double) * poisson_lds_doubles(ptn, lb, c->nrm);
       const long long M = (long long)c->nzm * c->sstr;
       const dim3 grid((unsigned)((M + lb - 1) / lb), (unsigned)n), block(lb, ptn);
       if (c->poisson_pq == 13)

@@ -86,7 +86,8 @@ typedef struct tc_tuning {
   int fft_t_lds;  /* LDS bytes of a theta-FFT block (default 32768) */
   int fft_z_lds;  /* LDS bytes of a z-FFT block (default 65536) */
   int fft_z_nt;   /* threads of a z-FFT block: 256 or 512 (default) */
-  int fft_pad;    /* LDS padding of the FFT tiles: 1 on (default), -1 off */
+  int fft_pad;    /* LDS padding of the FFT tiles: one slot per 8 points (1, default),
+                     per 4 (2) or per 16 (3); -1 off */
   int poisson_pq; /* rows per thread of the radial Poisson solve: 13 (default) or 26 */
 } tc_tuning;
 

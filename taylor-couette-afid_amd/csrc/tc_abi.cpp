@@ -286,7 +286,10 @@ tc_tuning resolve_tuning(const tc_tuning *in) {
   if (t.fft_t_lds == 0) t.fft_t_lds = env("TC_FFT_T_LDS", 32768);
   if (t.fft_z_lds == 0) t.fft_z_lds = env("TC_FFT_Z_LDS", 65536);
   if (t.fft_z_nt == 0) t.fft_z_nt = env("TC_FFT_Z_NT", 512) == 256 ? 256 : 512;
-  if (t.fft_pad == 0) t.fft_pad = env("TC_FFT_PAD", 1) == 0 ? -1 : 1;
+  if (t.fft_pad == 0) {
+    const int v = env("TC_FFT_PAD", 1);
+    t.fft_pad = (v == 0) ? -1 : std::max(1, std::min(3, v));
+  }
   if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", 13) == 26 ? 26 : 13;
   return t;
 }
@@ -299,7 +302,8 @@ std::string apply_plans(tc_ctx *c, const tc_tuning &t) {
   std::string e = tc::make_fft_plan(c->p.nthm, pt, t.fft_t_lds, 256);
   if (e.empty()) e = tc::make_fft_plan(c->p.nzm, pz, t.fft_z_lds, t.fft_z_nt == 256 ? 256 : 512);
   if (!e.empty()) return e;
-  if (t.fft_pad < 0) pt.padsh = pz.padsh = 31;
+  // one LDS padding slot per 8 (1), 4 (2) or 16 (3) points; -1: none
+  pt.padsh = pz.padsh = t.fft_pad < 0 ? 31 : (t.fft_pad == 2 ? 2 : (t.fft_pad == 3 ? 4 : 3));
   c->plan_t = pt;
   c->plan_z = pz;
   c->tuning = t;

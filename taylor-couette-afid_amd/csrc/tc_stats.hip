@@ -2,9 +2,10 @@
 // vorticity of CalcVorticity (StatRoutines.F90:12-71, CalcVorticity.F90:11-88)
 // and CalcDissipation (CalcDissipation.F90:11-153).
 //
-// One wave owns one (r-plane k, z-row j) theta line: its lanes stride along
-// theta, each lane sums its points, a fixed butterfly reduces the wave, and
-// the wave adds its sums to the (k, j) accumulators it alone owns.  So the
+// One 256-thread block owns one (r-plane k, z-row j) theta line: its
+// threads stride along theta, each sums its points, a fixed butterfly
+// reduces each wave, the four wave sums are added in order, and the block
+// adds the result to the (k, j) accumulators it alone owns.  So the
 // accumulators need no atomics and the sums are the same run to run.  The
 // vorticity is evaluated at the point and never stored.
 //
@@ -26,13 +27,15 @@ struct Fields {
 constexpr int NS = TC_NSTATS;  // accumulators
 constexpr int NV = NS + 1;     // + the pvol-weighted dissipation of nute
 
+// One 256-thread block per (k, j) theta line: lanes stride along theta by
+// 256, each wave reduces with a fixed butterfly, and the four wave sums are
+// added in wave order.
 template <bool STATS, bool DISS>
 __global__ __launch_bounds__(256) void k_stats(GridDev g, Fields f, double *__restrict__ acc,
                                                double *__restrict__ part, double ren) {
-  const int lane = threadIdx.x & 63;
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = blockIdx.x;
   const int k = blockIdx.y + 1;
-  if (j >= g.nzm) return;
   const int nthm = g.nthm, nzm = g.nzm, nrm = g.nrm;
   const long long P = g.P;
   const int kp = k + 1;                       // qr face above the cell
@@ -54,7 +57,7 @@ __global__ __launch_bounds__(256) void k_stats(GridDev g, Fields f, double *__re
 #pragma unroll
   for (int q = 0; q < NV; q++) s[q] = 0.0;
 
-  for (int i = lane; i < nthm; i += 64) {
+  for (int i = threadIdx.x; i < nthm; i += 256) {
     const int ip = i + 1 == nthm ? 0 : i + 1, im = i == 0 ? nthm - 1 : i - 1;
     if (STATS) {
       // CalcVorticity.F90:41-79
@@ -115,23 +118,29 @@ __global__ __launch_bounds__(256) void k_stats(GridDev g, Fields f, double *__re
       s[NS] += d * pvol * pi;
     }
   }
-  // fixed butterfly over the wave
+  // fixed butterfly over each wave, then the four waves in order
+  __shared__ double sh[4][NV];
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double v = s[q];
 #pragma unroll
     for (int w = 32; w >= 1; w >>= 1) v += __shfl_xor(v, w, 64);
-    s[q] = v;
+    if (lane == 0) sh[wave][q] = v;
   }
-  // lane q adds accumulator q of this (k, j)
+  __syncthreads();
+  // thread q adds accumulator q of this (k, j)
   const long long kj = (long long)(k - 1) + (long long)nrm * j;
   const long long M = (long long)nrm * nzm;
-#pragma unroll
-  for (int q = 0; q < NS; q++) {
-    const bool mine = (q == TC_ST_DISSTE) ? DISS : STATS;
-    if (mine && lane == q) acc[q * M + kj] += s[q];
+  const int q = threadIdx.x;
+  if (q < NV) {
+    const double v = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+    if (q < NS) {
+      const bool mine = (q == TC_ST_DISSTE) ? DISS : STATS;
+      if (mine) acc[q * M + kj] += v;
+    } else if (DISS) {
+      part[kj] = v;
+    }
   }
-  if (DISS && lane == 0) part[kj] = s[NS];
 }
 
 // Sum of the n per-line dissipation partials in a fixed order (one block).
@@ -222,7 +231,7 @@ void launch_dump(tc_ctx *c, int kind, int k) {
 // vorticity); diss: CalcDissipation, whose volume sum lands in c->red_out[0].
 void launch_stats(tc_ctx *c, bool stats, bool diss) {
   timer_begin(c, "stats");
-  const dim3 grid((unsigned)((c->nzm + 3) / 4), (unsigned)c->nrm);
+  const dim3 grid((unsigned)c->nzm, (unsigned)c->nrm);
   Fields f{c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR]};
   const double ren = c->p.ren;
   if (stats && diss)

@@ -184,6 +184,9 @@ int tc_synchronize(tc_ctx *ctx);
  * each launch of a timed kernel is bracketed by two events; read returns the
  * summed milliseconds and launch count since the last reset. */
 int tc_kernel_timing(tc_ctx *ctx, int enable);
+/* Time only the launches named kernel (NULL: timing off); fewer events in
+ * the stream when one kernel's duration is wanted inside a timed region. */
+int tc_kernel_timing_only(tc_ctx *ctx, const char *kernel);
 int tc_kernel_timing_read(tc_ctx *ctx, const char *kernel, double *total_ms,
                           long long *count, int reset);
 

@@ -63,7 +63,7 @@ ABI_SYMBOLS = [
     "tc_write_restart", "tc_read_restart",
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
-    "tc_regrid_field", "tc_stream_triad",
+    "tc_regrid_field", "tc_stream_triad", "tc_kernel_timing_only",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -145,6 +145,7 @@ def lib():
     L.tc_init_perturbed_couette.argtypes = [ctypes.POINTER(Params), ctypes.c_ulonglong,
                                             ctypes.c_double, _D, _D, _D]
     L.tc_kernel_timing.argtypes = [_V, ctypes.c_int]
+    L.tc_kernel_timing_only.argtypes = [_V, ctypes.c_char_p]
     L.tc_kernel_timing_read.argtypes = [_V, ctypes.c_char_p, _D,
                                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
     L.tc_device_bytes.argtypes = [_V, ctypes.POINTER(ctypes.c_ulonglong)]
@@ -427,8 +428,12 @@ class Solver:
                                "vmax1", "vmax2", "vmax3"], o))
 
     # --- instrumentation ---
-    def kernel_timing(self, on):
-        _check(lib().tc_kernel_timing(self.h, int(bool(on))))
+    def kernel_timing(self, on, only=None):
+        """per-kernel HIP-event timing; only: time just that kernel name"""
+        if only:
+            _check(lib().tc_kernel_timing_only(self.h, only.encode()))
+        else:
+            _check(lib().tc_kernel_timing(self.h, int(bool(on))))
 
     def kernel_time(self, name, reset=False):
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()

@@ -136,7 +136,9 @@ void build_qr_table(const tc::Grid &g, double betadx, double *T, int nt) {
 
 int refresh_tables(tc_ctx *c) {
   const int nsst = c->p.nsst;
-  bool any = !c->lu_valid;
+  // with every wall mask 1 only the no-slip variant (3) is ever read
+  const bool all_variants = !c->masks_uniform_one;
+  bool any = !c->lu_valid || (all_variants && !c->lu_all_variants);
   for (int s = 0; s < nsst; s++)
     if (c->lu_key[s] != c->beta * c->g.alm[s + 1]) any = true;
   if (!any) return 0;
@@ -146,7 +148,7 @@ int refresh_tables(tc_ctx *c) {
   for (int s = 0; s < nsst; s++) {
     const double betadx = c->beta * c->g.alm[s + 1];
     double *base = c->lu_host + (size_t)s * per_slot;
-    for (int v = 0; v < 4; v++) {
+    for (int v = all_variants ? 0 : 3; v < 4; v++) {
       build_wall_table(c->g, tc::LU_QT, v & 1, v >> 1, betadx,
                        base + ((size_t)tc::LU_QT * 4 + v) * c->lu_nt, c->lu_nt);
       build_wall_table(c->g, tc::LU_QZ, v & 1, v >> 1, betadx,
@@ -159,6 +161,7 @@ int refresh_tables(tc_ctx *c) {
                         c->stream));
   HIPCHK(hipEventRecord(c->lu_copied, c->stream));
   c->lu_valid = true;
+  c->lu_all_variants = all_variants;
   return 0;
 }
 
@@ -338,7 +341,10 @@ int set_error(const std::string &msg) { return fail(msg); }
 int check_launch() { return launch_status(); }
 
 void timer_begin(tc_ctx *c, const char *name) {
+  c->timer_open = false;
   if (!c->timing) return;
+  if (!c->timing_only.empty() && c->timing_only != name) return;
+  c->timer_open = true;
   Timer t;
   t.name = name;
   for (int q = 0; q < 2; q++) {
@@ -356,8 +362,9 @@ void timer_begin(tc_ctx *c, const char *name) {
 }
 
 void timer_end(tc_ctx *c) {
-  if (!c->timing || c->timers.empty()) return;
+  if (!c->timer_open || c->timers.empty()) return;
   hipEventRecord(c->timers.back().b, c->stream);
+  c->timer_open = false;
 }
 
 }  // namespace tc
@@ -829,7 +836,10 @@ int tc_set_masks(tc_ctx *c, const int *icylsl, const int *ocylsl) {
       return fail("masks must hold 0 or 1");
   c->icyl.assign(icylsl, icylsl + n);
   c->ocyl.assign(ocylsl, ocylsl + n);
-  return upload_masks(c);
+  const int rc = upload_masks(c);
+  if (rc) return rc;
+  // slip walls need the other three LU variants of the current tables
+  return c->lu_valid ? refresh_tables(c) : 0;
 }
 
 int tc_get_masks(tc_ctx *c, int *icylsl, int *ocylsl) {
@@ -1177,6 +1187,14 @@ int tc_synchronize(tc_ctx *c) {
 int tc_kernel_timing(tc_ctx *c, int enable) {
   CTXCHK(c);
   c->timing = enable != 0;
+  c->timing_only.clear();
+  return 0;
+}
+
+int tc_kernel_timing_only(tc_ctx *c, const char *kernel) {
+  CTXCHK(c);
+  c->timing = kernel != nullptr;
+  c->timing_only = kernel ? kernel : "";
   return 0;
 }
 

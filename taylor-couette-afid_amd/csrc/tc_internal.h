@@ -135,6 +135,7 @@ struct tc_ctx {
   int lu_nt = 0;               // doubles per table = LU_ROW*(nrm+1)
   double lu_key[3] = {-1, -1, -1};
   bool lu_valid = false;
+  bool lu_all_variants = false;  // all four mask variants built (else only variant 3)
   // FFT
   tc::FFTPlan plan_t, plan_z;
   double2 *tw_t = nullptr, *tw_z = nullptr;
@@ -161,6 +162,8 @@ struct tc_ctx {
   double cfl_cached = 0.0;
   // timing
   bool timing = false;
+  bool timer_open = false;   // the last timer_begin recorded a start event
+  std::string timing_only;   // when set, only this kernel name is timed
   std::vector<tc::Timer> timers;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, long long>> timing_acc;

@@ -332,9 +332,15 @@ def _kernel_short(name):
     return name.split("::", 1)[1] if "::" in name else name
 
 
+# device kernels whose timer name differs from the kernel name
+_KERNEL_TIMER = {"k_correct_both2": "correct_both"}
+
+
 def _timer_of(kernel):
     """Kernel timer name (the kernel_bytes key) of a device kernel name."""
     base = kernel.split("<")[0]
+    if base in _KERNEL_TIMER:
+        return _KERNEL_TIMER[base]
     return base[2:] if base.startswith("k_") else base
 
 

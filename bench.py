@@ -109,14 +109,14 @@ def gpu_leg(tc, p, seed, amp, steps, warmup):
     dev.kernel_time("", reset=True)
     dev.synchronize()
     t0 = time.perf_counter()
-    last = None
-    for _ in range(steps):
-        code, last = dev.step(ntime)
-        ntime += 1
-        if code not in (0, 555, 333):
-            raise RuntimeError(f"step {ntime} ended with error code {code}: {last}")
+    # the K main-loop iterations run in the library (tc_run_steps), each with
+    # its CFL, dt rule, RK3 step, logging pass and error codes
+    code, n_last, last = dev.run(ntime, steps)
     dev.synchronize()
     t1 = time.perf_counter()
+    if code != 0 or n_last != ntime + steps - 1:
+        raise RuntimeError(f"timed steps ended at ntime {n_last} with error code {code}: {last}")
+    ntime += steps
     dom_ms, dom_cnt = dev.kernel_time(DOMINANT, reset=True)
     # per-kernel table from a few more steps with every kernel timed
     dev.kernel_timing(True)

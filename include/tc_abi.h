@@ -166,6 +166,15 @@ int tc_calc_velocity_stats(tc_ctx *ctx, double *vmax3, double *rms3); /* CalcVel
  * Returns the error code (>= 0) or a negative ABI error. */
 int tc_step(tc_ctx *ctx, int ntime, double walltime, double *out9);
 
+/* The main loop from ntime0 for at most nsteps iterations of tc_step, ending
+ * at the first nonzero error code (main.F90:152-261 without the host
+ * between steps).  walltime_start is the wall-clock origin of WALLTIMEMAX
+ * (seconds on the same clock as the caller's).  On return *ntime_last is
+ * the last iteration run and out9 its diagnostics; the result is that
+ * iteration's error code, or a negative ABI error. */
+int tc_run_steps(tc_ctx *ctx, int ntime0, int nsteps, double walltime_start, int *ntime_last,
+                 double *out9);
+
 /* Divergence-free projection of the current velocity with al = dt = 1
  * (CalcLocalDivergence + SolvePressureCorrection + CorrectVelocity), used
  * to build starting fields; pr is left untouched. */

@@ -63,7 +63,7 @@ ABI_SYMBOLS = [
     "tc_write_restart", "tc_read_restart",
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
-    "tc_regrid_field", "tc_stream_triad", "tc_kernel_timing_only",
+    "tc_regrid_field", "tc_stream_triad", "tc_kernel_timing_only", "tc_run_steps",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -142,6 +142,7 @@ def lib():
     L.tc_calc_torque.argtypes = [_V, _D, _D]
     L.tc_calc_velocity_stats.argtypes = [_V, _D, _D]
     L.tc_step.argtypes = [_V, ctypes.c_int, ctypes.c_double, _D]
+    L.tc_run_steps.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I, _D]
     L.tc_init_perturbed_couette.argtypes = [ctypes.POINTER(Params), ctypes.c_ulonglong,
                                             ctypes.c_double, _D, _D, _D]
     L.tc_kernel_timing.argtypes = [_V, ctypes.c_int]
@@ -426,6 +427,16 @@ class Solver:
         code = _check(lib().tc_step(self.h, ntime, walltime, _dp(o)))
         return code, dict(zip(["cfl", "dt", "time", "dmax", "torque_in", "torque_out",
                                "vmax1", "vmax2", "vmax3"], o))
+
+    def run(self, ntime0, nsteps, walltime=0.0):
+        """tc_step from ntime0 for at most nsteps iterations, stopping at the
+        first error code; returns (errorcode, last ntime, diagnostics)."""
+        o = np.zeros(9)
+        n = ctypes.c_int()
+        code = _check(lib().tc_run_steps(self.h, int(ntime0), int(nsteps), float(walltime),
+                                         ctypes.byref(n), _dp(o)))
+        return code, n.value, dict(zip(["cfl", "dt", "time", "dmax", "torque_in", "torque_out",
+                                        "vmax1", "vmax2", "vmax3"], o))
 
     # --- instrumentation ---
     def kernel_timing(self, on, only=None):

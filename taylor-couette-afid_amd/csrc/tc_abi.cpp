@@ -15,6 +15,8 @@
 
 #include "tc_internal.h"
 
+#include <chrono>
+
 namespace {
 
 thread_local std::string g_err;
@@ -1138,6 +1140,23 @@ int tc_last_dissipation(tc_ctx *c, double *nute) {
   CTXCHK(c);
   if (nute) *nute = c->last_nute;
   return 0;
+}
+
+int tc_run_steps(tc_ctx *c, int ntime0, int nsteps, double walltime_start, int *ntime_last,
+                 double *o) {
+  CTXCHK(c);
+  if (nsteps < 1) return fail("tc_run_steps needs nsteps >= 1");
+  const auto t0 = std::chrono::steady_clock::now();
+  int code = 0, n = ntime0;
+  for (int q = 0; q < nsteps; q++) {
+    n = ntime0 + q;
+    const double wall =
+        walltime_start + std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    code = tc_step(c, n, wall, o);
+    if (code != 0) break;
+  }
+  if (ntime_last) *ntime_last = n;
+  return code;
 }
 
 int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed, double amp, double *qt,

@@ -1203,10 +1203,21 @@ int tc_synchronize(tc_ctx *c) {
   return 0;
 }
 
+// Events are taken from a pool; creating them while the stream is idle at a
+// step boundary would sit on the critical path, so the pool is filled here.
+static void fill_event_pool(tc_ctx *c, size_t n) {
+  while (c->event_pool.size() < n) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) break;
+    c->event_pool.push_back(e);
+  }
+}
+
 int tc_kernel_timing(tc_ctx *c, int enable) {
   CTXCHK(c);
   c->timing = enable != 0;
   c->timing_only.clear();
+  if (c->timing) fill_event_pool(c, 1024);
   return 0;
 }
 
@@ -1214,6 +1225,7 @@ int tc_kernel_timing_only(tc_ctx *c, const char *kernel) {
   CTXCHK(c);
   c->timing = kernel != nullptr;
   c->timing_only = kernel ? kernel : "";
+  if (c->timing) fill_event_pool(c, 1024);
   return 0;
 }
 

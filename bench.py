@@ -72,7 +72,7 @@ def kernel_bytes(p):
         "correct_velocity": 7 * R, "correct_pressure": 3 * R,
         "max_cfl": 3 * R, "max_div": 3 * R,
         "helm_fwd": 13 * R,          # qt qr qz pr ru1 ru2 ru3 | ru1 ru2 ru3 yt yr yz
-        "helm_bwd": 9 * R,           # yt yr yz qt qr qz | qt qr qz
+        "helm_bwd": 6 * R,           # yt yr yz (U q + y) | qt qr qz
         "diagnostics": 3 * R,        # qt qr qz -> 10 scalars
     }
 

@@ -71,6 +71,7 @@ void write_table(double *T, int nt, int n, std::vector<double> &dl, std::vector<
   for (int q = 0; q < n; q++) {
     T[W * q + 3] = 1.0 / d[q];
     T[W * q + 4] = sc[q];
+    T[W * q + 6] = d[q];
   }
 }
 

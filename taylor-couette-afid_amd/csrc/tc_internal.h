@@ -57,7 +57,8 @@ struct FFTPlan {
 };
 
 // Helmholtz LU tables: per (substep slot, component, mask variant) a block of
-// NT rows of LU_ROW doubles: dl, du, du2, rd=1/d, row scale, pivot flag, 0, 0.
+// NT rows of LU_ROW doubles: dl, du, du2, rd=1/d, row scale, pivot flag, d, 0
+// (d, du, du2 are the rows of the upper factor U).
 // One row is one 64-byte scalar load.
 enum { LU_QT = 0, LU_QR = 1, LU_QZ = 2 };
 constexpr int LU_ROW = 8;

@@ -200,6 +200,11 @@ struct LUView {
   DEVI double rd(int r) const { return b[LU_ROW * r + 3]; }
   DEVI double sc(int r) const { return b[LU_ROW * r + 4]; }
   DEVI double pv(int r) const { return b[LU_ROW * r + 5]; }
+  DEVI double ud(int r) const { return b[LU_ROW * r + 6]; }
+  // row r of U q for q_r, q_(r+1), q_(r+2)
+  DEVI double uq(int r, double q0, double q1, double q2) const {
+    return ud(r) * q0 + du(r) * q1 + du2(r) * q2;
+  }
 };
 DEVI LUView lu_view(const double *base, int nt) {
   LUView v;
@@ -395,13 +400,23 @@ __global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a,
 // Backward substitution (dgtts2) of one line: q += x, rows first..n-1 (the
 // q_r wall rows are reset instead).  Loads are issued a chunk of rows ahead
 // of the dependent recurrence.
+// FOLD: y already holds U q + y (the forward sweep added it), so the
+// solution of U x' = y' is x' = q + x itself and q is not read.
+template <bool FOLD>
 DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
                       const double *__restrict__ y, long long l, long long P) {
   constexpr int CH = 8;
+  auto put = [&](long long o, double qv, double x) { q[o] = FOLD ? x : qv + x; };
   double x1 = y[(long long)(n - 1) * P + l] * T.rd(n - 1);
-  if (n - 1 >= first) q[(long long)(n - 1) * P + l] += x1;
+  if (n - 1 >= first) {
+    const long long o = (long long)(n - 1) * P + l;
+    put(o, FOLD ? 0.0 : q[o], x1);
+  }
   double x0 = (y[(long long)(n - 2) * P + l] - T.du(n - 2) * x1) * T.rd(n - 2);
-  if (n - 2 >= first) q[(long long)(n - 2) * P + l] += x0;
+  if (n - 2 >= first) {
+    const long long o = (long long)(n - 2) * P + l;
+    put(o, FOLD ? 0.0 : q[o], x0);
+  }
   for (int r0 = n - 3; r0 >= first; r0 -= CH) {
     double yv[CH], qv[CH];
 #pragma unroll
@@ -410,7 +425,7 @@ DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
       if (r >= first) {
         const long long o = (long long)r * P + l;
         yv[u] = y[o];
-        qv[u] = q[o];
+        qv[u] = FOLD ? 0.0 : q[o];
       }
     }
 #pragma unroll
@@ -418,7 +433,7 @@ DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
       const int r = r0 - u;
       if (r >= first) {
         const double x = (yv[u] - T.du(r) * x0 - T.du2(r) * x1) * T.rd(r);
-        q[(long long)r * P + l] = qv[u] + x;
+        put((long long)r * P + l, qv[u], x);
         x1 = x0;
         x0 = x;
       }
@@ -427,6 +442,7 @@ DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
 }
 
 // which: bit 0 q_theta, bit 1 q_r, bit 2 q_z
+template <bool FOLD>
 DEVI void helm_bwd_body(const long long l, const GridDev &g, double *__restrict__ qt,
                         double *__restrict__ qr, double *__restrict__ qz,
                         const double *__restrict__ yt, const double *__restrict__ yr,
@@ -436,17 +452,18 @@ DEVI void helm_bwd_body(const long long l, const GridDev &g, double *__restrict_
                         int which) {
   if (l >= g.P) return;
   const int v = icyl[l] + 2 * ocyl[l];
-  if (which & 1) usolve_line(lu_view(lut + (size_t)v * nt, nt), g.nrm, 0, qt, yt, l, g.P);
+  if (which & 1) usolve_line<FOLD>(lu_view(lut + (size_t)v * nt, nt), g.nrm, 0, qt, yt, l, g.P);
   if (which & 2) {
     // SolveImpEqnUpdate_QR.F90:63-65 adds rows 2..nrm; walls reset
     // (ImplicitAndUpdateQR.F90:70-71)
-    usolve_line(lu_view(lur, nt), g.nr, 1, qr, yr, l, g.P);
+    usolve_line<FOLD>(lu_view(lur, nt), g.nr, 1, qr, yr, l, g.P);
     qr[l] = 0.0;
     qr[(long long)(g.nr - 1) * g.P + l] = 0.0;
   }
-  if (which & 4) usolve_line(lu_view(luz + (size_t)v * nt, nt), g.nrm, 0, qz, yz, l, g.P);
+  if (which & 4) usolve_line<FOLD>(lu_view(luz + (size_t)v * nt, nt), g.nrm, 0, qz, yz, l, g.P);
 }
 
+template <bool FOLD>
 __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict__ qt,
                                                   double *__restrict__ qr,
                                                   double *__restrict__ qz,
@@ -459,14 +476,14 @@ __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict_
                                                   const double *__restrict__ lur,
                                                   const double *__restrict__ luz, int nt,
                                                   int which) {
-  helm_bwd_body((long long)blockIdx.x * blockDim.x + threadIdx.x, g, qt, qr, qz, yt, yr, yz, icyl,
-                ocyl, lut, lur, luz, nt, which);
+  helm_bwd_body<FOLD>((long long)blockIdx.x * blockDim.x + threadIdx.x, g, qt, qr, qz, yt, yr, yz,
+                      icyl, ocyl, lut, lur, luz, nt, which);
 }
 
-// Grouped fused backward sweep: blockIdx.y = instance; y = rhs, dq, qcap.
+// Grouped fused backward sweep: blockIdx.y = instance; y' = rhs, dq, qcap.
 __global__ __launch_bounds__(256) void k_helm_bwd_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
-  helm_bwd_body((long long)blockIdx.x * blockDim.x + threadIdx.x, d.g, d.f[TC_QT], d.f[TC_QR],
+  helm_bwd_body<true>((long long)blockIdx.x * blockDim.x + threadIdx.x, d.g, d.f[TC_QT], d.f[TC_QR],
                 d.f[TC_QZ], d.f[TC_RHS], d.f[TC_DQ], d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur,
                 d.luz, d.lu_nt, 7);
 }
@@ -621,7 +638,7 @@ DEVI void helm_fwd_body(
           ct[u] = (r + a.bc1) * Tt[u].sc(0);
         } else {
           lsolve_step(Tt[u], k - 2, r * Tt[u].sc(k - 1), ct[u], yv);
-          yt[o - P] = yv;
+          yt[o - P] = yv + Tt[u].uq(k - 2, qm, T(0, 0, 0), qp);
         }
       }
       // q_r (rows 2..nrm; row 1 is the wall with zero right-hand side)
@@ -630,7 +647,7 @@ DEVI void helm_fwd_body(
         const double r = rhs_qr(hc, a, R(-1, 0, 0), R(0, 0, 0), R(1, 0, 0), pr_c, pr_km[u], dphv, h2[u]);
         ru2[o] = dphv;
         lsolve_step(Tr, k - 2, r * Tr.sc(k - 1), cr[u], yv);
-        yr[o - P] = yv;
+        yr[o - P] = yv + Tr.uq(k - 2, R(-1, 0, 0), R(0, 0, 0), R(1, 0, 0));
       }
       // q_z
       {
@@ -642,7 +659,7 @@ DEVI void helm_fwd_body(
           cz[u] = r * Tz[u].sc(0);
         } else {
           lsolve_step(Tz[u], k - 2, r * Tz[u].sc(k - 1), cz[u], yv);
-          yz[o - P] = yv;
+          yz[o - P] = yv + Tz[u].uq(k - 2, qm, Z(0, 0, 0), qp);
         }
       }
       pr_km[u] = pr_c;
@@ -668,15 +685,18 @@ DEVI void helm_fwd_body(
     }
     __syncthreads();
   }
+  // after the last barrier the ring still holds planes nrm and nrm+1
+  const int s0 = nrm % 3, s1 = (nrm + 1) % 3;
 #pragma unroll
   for (int u = 0; u < LPT; u++) {
     if (!valid[u]) continue;
-    yt[(long long)(nrm - 1) * P + l[u]] = ct[u];
-    yz[(long long)(nrm - 1) * P + l[u]] = cz[u];
+    const int b = base[u];
+    yt[(long long)(nrm - 1) * P + l[u]] = ct[u] + Tt[u].ud(nrm - 1) * sq[0][s0][b];
+    yz[(long long)(nrm - 1) * P + l[u]] = cz[u] + Tz[u].ud(nrm - 1) * sq[2][s0][b];
     double c2 = cr[u];
     lsolve_step(Tr, g.nr - 2, 0.0, c2, yv);  // wall row nr
-    yr[(long long)(g.nr - 2) * P + l[u]] = yv;
-    yr[(long long)(g.nr - 1) * P + l[u]] = c2;
+    yr[(long long)(g.nr - 2) * P + l[u]] = yv + Tr.ud(g.nr - 2) * sq[1][s0][b] + Tr.du(g.nr - 2) * sq[1][s1][b];
+    yr[(long long)(g.nr - 1) * P + l[u]] = c2 + Tr.ud(g.nr - 1) * sq[1][s1][b];
   }
 }
 
@@ -733,7 +753,7 @@ static unsigned line_blocks(const tc_ctx *c) { return (unsigned)((c->P + 255) / 
 
 static void launch_bwd(tc_ctx *c, int which, const char *name) {
   timer_begin(c, name);
-  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_helm_bwd<true>, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_DQ],
                      c->f[TC_QCAP], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
                      lu_slot(c, LU_QZ), c->lu_nt, which);
@@ -748,7 +768,7 @@ void launch_implicit_qt(tc_ctx *c) {
                      imp_args(c), c->f[TC_QT], c->f[TC_PR], c->f[TC_DQ], c->f[TC_RU1],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), c->lu_nt);
   timer_end(c);
-  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_helm_bwd<false>, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_RHS],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
                      lu_slot(c, LU_QZ), c->lu_nt, 1);
@@ -759,7 +779,7 @@ void launch_implicit_qr(tc_ctx *c) {
                      imp_args(c), c->f[TC_QR], c->f[TC_PR], c->f[TC_DPH], c->f[TC_RU2],
                      c->f[TC_RHS], lu_slot(c, LU_QR), c->lu_nt);
   timer_end(c);
-  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_helm_bwd<false>, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_RHS],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
                      lu_slot(c, LU_QZ), c->lu_nt, 2);
@@ -770,7 +790,7 @@ void launch_implicit_qz(tc_ctx *c) {
                      imp_args(c), c->f[TC_QZ], c->f[TC_PR], c->f[TC_QCAP], c->f[TC_RU3],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QZ), c->lu_nt);
   timer_end(c);
-  hipLaunchKernelGGL(k_helm_bwd, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
+  hipLaunchKernelGGL(k_helm_bwd<false>, dim3(line_blocks(c)), dim3(256), 0, c->stream, c->gd,
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_RHS], c->f[TC_RHS],
                      c->f[TC_RHS], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
                      lu_slot(c, LU_QZ), c->lu_nt, 4);

@@ -34,7 +34,7 @@ PROFILES = os.path.join(ROOT, "profiles")
 # summaries (run from /tmp with TMPDIR=/tmp; OUT is the output folder)
 _LEG = "--no-cpu --sweep-steps 0 --served-jobs 0"
 # the kernel the roofline object reports (the largest share of the step)
-DOMINANT = "helm_fwd"
+DOMINANT = "helm_fwd_corr"
 
 PROFILE_COMMANDS = {
     "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o bench "
@@ -72,6 +72,8 @@ def kernel_bytes(p):
         "correct_velocity": 7 * R, "correct_pressure": 3 * R,
         "max_cfl": 3 * R, "max_div": 3 * R,
         "helm_fwd": 13 * R,          # qt qr qz pr ru1 ru2 ru3 | ru1 ru2 ru3 yt yr yz
+        # substeps 2 and 3: the previous substep's correction applied on the fly
+        "helm_fwd_corr": 15 * R,     # qt qr qz pr dph ru1 ru2 ru3 | ru1 ru2 ru3 yt yr yz pr
         "helm_bwd": 6 * R,           # yt yr yz (U q + y) | qt qr qz
         "diagnostics": 3 * R,        # qt qr qz -> 10 scalars
     }
@@ -341,6 +343,8 @@ def _timer_of(kernel):
     base = kernel.split("<")[0]
     if base in _KERNEL_TIMER:
         return _KERNEL_TIMER[base]
+    if base == "k_helm_fwd" and kernel.replace(" ", "").endswith(",true>"):
+        return "helm_fwd_corr"  # k_helm_fwd<UNIFORM, LPT, CORR = true>
     return base[2:] if base.startswith("k_") else base
 
 
@@ -420,11 +424,16 @@ def traffic_of(workload, timer):
     path = os.path.join(PROFILES, f"all_kernels_{workload}_hbm_traffic.md")
     if not os.path.exists(path):
         return None
+    rows = {}
     for line in open(path):
         cells = [c.strip() for c in line.split("|")[1:-1]]
-        if len(cells) == 7 and cells[0].startswith("k_") and _timer_of(cells[0]) in (timer, timer.rsplit("_", 1)[0]):
+        if len(cells) == 7 and cells[0].startswith("k_"):
+            rows.setdefault(_timer_of(cells[0]), cells[4])
+    # exact timer first, then the kernel shared by a forward/inverse pair
+    for t in (timer, timer.rsplit("_", 1)[0]):
+        if t in rows:
             try:
-                return int(float(cells[4]))
+                return int(float(rows[t]))
             except ValueError:
                 return None
     return None

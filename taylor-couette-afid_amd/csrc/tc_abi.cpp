@@ -205,6 +205,7 @@ void destroy_ctx(tc_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int q = 0; q < TC_NFIELDS; q++)
     if (c->f[q]) hipFree(c->f[q]);
+  if (c->pr_alt) hipFree(c->pr_alt);
   if (c->spec) hipFree(c->spec);
   if (c->gvbuf) hipFree(c->gvbuf);
   if (c->mask) hipFree(c->mask);
@@ -234,8 +235,20 @@ int read_reduction(tc_ctx *c, int nv, double *out) {
   return 0;
 }
 
+// Deferred correction (fused RK3 with an even nthm): substep 2 reads the
+// uncorrected pressure from pr and writes the corrected one to pr_alt,
+// substep 3 the other way round, and the last substep's correction is a
+// pass of its own, so every step ends with its fields where the per-substep
+// schedule leaves them.  The two schedules agree to rounding: the corrected
+// values are the same bits, but the compiler forms the Helmholtz kernel's
+// multiply-adds differently in its two variants.
 int run_substep(tc_ctx *c) {
-  if (c->fused) {
+  const bool defer = c->fused && ::tc::defers_correction(c);
+  if (c->fused && defer && c->ns > 1) {
+    const bool even = (c->ns % 2 == 0);
+    ::tc::launch_helmholtz_fused_corr(c, even ? c->f[TC_PR] : c->pr_alt,
+                                    even ? c->pr_alt : c->f[TC_PR], c->corr_al);
+  } else if (c->fused) {
     tc::launch_helmholtz_fused(c);
   } else {
     tc::launch_explicit_qt(c);
@@ -251,7 +264,9 @@ int run_substep(tc_ctx *c) {
     tc::launch_divergence(c, 1.0 / (c->al * c->dt));
     tc::launch_poisson(c);
   }
-  if (c->fused) {
+  if (c->fused && defer && c->ns < c->p.nsst) {
+    c->corr_al = c->al;  // applied by the next substep's Helmholtz kernel
+  } else if (c->fused) {
     tc::launch_correct_both(c);
   } else {
     tc::launch_correct_velocity(c, c->al);
@@ -332,6 +347,12 @@ void torque_from_sums(const tc_ctx *c, double slow, double sup, double *inner, d
 }
 
 }  // namespace
+
+namespace tc {
+bool defers_correction(const tc_ctx *c) {
+  return c->defer_corr && c->pr_alt && c->p.nsst == 3 && c->nthm % 2 == 0;
+}
+}  // namespace tc
 
 namespace tc {
 
@@ -437,6 +458,14 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   for (int q = 0; q < TC_NFIELDS; q++) {
     TRY(dev_alloc(c, (void **)&c->f[q], sizeof(double) * (size_t)c->N));
     HTRY(hipMemsetAsync(c->f[q], 0, sizeof(double) * (size_t)c->N, c->stream));
+  }
+  {
+    const char *e = std::getenv("TC_DEFER_CORRECTION");
+    c->defer_corr = !(e && std::atoi(e) == 0);
+  }
+  if (c->defer_corr) {
+    TRY(dev_alloc(c, (void **)&c->pr_alt, sizeof(double) * (size_t)c->N));
+    HTRY(hipMemsetAsync(c->pr_alt, 0, sizeof(double) * (size_t)c->N, c->stream));
   }
   const size_t M = (size_t)c->nzm * c->sstr;
   TRY(dev_alloc(c, (void **)&c->spec, sizeof(double2) * M * c->nrm));

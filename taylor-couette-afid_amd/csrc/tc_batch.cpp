@@ -57,12 +57,13 @@ struct Group {
   size_t dyn_off = 0;  // offset of this group's entries in the staging slot
 };
 
-using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int>;
+using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int, int>;
 
 Key group_key(const tc_ctx *c) {
   return Key(c->nthm, c->nzm, c->nrm, c->p.nsst, c->helm_lpt, c->poisson_pq,
              c->masks_uniform_one ? 1 : 0, c->plan_t.lines, c->plan_t.me, c->plan_t.padsh,
-             c->plan_z.lines, c->plan_z.me * 1000 + c->plan_z.nt, c->plan_z.padsh);
+             c->plan_z.lines, c->plan_z.me * 1000 + c->plan_z.nt, c->plan_z.padsh,
+             tc::defers_correction(c) ? 1 : 0);
 }
 
 #define BCHK(x)                                                                  \
@@ -195,6 +196,12 @@ StepDesc make_desc(tc_ctx *c, int ns) {
   d.twz = c->tw_z;
   d.red = c->red;
   d.red_out = c->red_out;
+  // deferred correction: the same alternation of pr and pr_alt as the
+  // single-instance schedule
+  d.corr = (tc::defers_correction(c) && ns > 1) ? 1 : 0;
+  d.corr_al = ns > 1 ? g.alm[ns - 1] : 0.0;
+  d.pr_in = (d.corr && ns % 2 == 1) ? c->pr_alt : c->f[TC_PR];
+  d.pr_out = d.corr ? (ns % 2 == 0 ? c->pr_alt : c->f[TC_PR]) : nullptr;
   d.limitcfl = c->p.limitcfl;
   d.resid = c->p.resid;
   d.limitvel = c->p.limitvel;
@@ -280,9 +287,10 @@ int enqueue_step(tc_batch *b) {
     BCHK(hipStreamWaitEvent(g.stream, b->fork, 0));
     tc::launch_cfl_check_g(c, g.tab[0], g.dyn, g.res, n, g.stream);
     for (int ns = 1; ns <= c->p.nsst; ns++) {
-      tc::launch_helmholtz_fused_g(c, g.tab[ns - 1], g.dyn, ns, n, g.stream);
+      const bool defer = tc::defers_correction(c);
+      tc::launch_helmholtz_fused_g(c, g.tab[ns - 1], g.dyn, ns, n, g.stream, defer && ns > 1);
       tc::launch_poisson_g(c, g.tab[ns - 1], n, g.stream);
-      tc::launch_correct_both_g(c, g.tab[ns - 1], n, g.stream);
+      if (!defer || ns == c->p.nsst) tc::launch_correct_both_g(c, g.tab[ns - 1], n, g.stream);
     }
     tc::launch_diagnostics_g(c, g.tab[0], g.dyn, g.res, n, g.stream);
     BCHK(hipEventRecord(g.join, g.stream));

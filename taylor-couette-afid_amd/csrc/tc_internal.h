@@ -93,6 +93,13 @@ struct StepDesc {
   double *red, *red_out;        // reduction partials and results
   double limitcfl, resid, limitvel;
   double virotint, virotext, del, deln, pi;
+  // deferred pressure correction (corr != 0): the Helmholtz kernel applies
+  // the previous substep's correction (al corr_al) reading pr_in and writing
+  // pr_out; otherwise pr_in is the pressure field
+  int corr;
+  double corr_al;
+  const double *pr_in;
+  double *pr_out;
 };
 
 // Per-instance scalars that change every step (written by the host before
@@ -149,6 +156,13 @@ struct tc_ctx {
   double time = 0, dt = 0, beta = 0, al = 0, ga = 0, ro = 0, vinner_t = 0, dvinner_t = 0;
   int ns = 1;
   bool fused = true;
+  // fused RK3 step: the pressure correction of substeps 1 and 2 is applied
+  // by the next substep's Helmholtz kernel instead of its own pass; pr_alt
+  // receives the corrected pressure of substep 1 (TC_DEFER_CORRECTION=0
+  // at tc_create turns this off)
+  bool defer_corr = true;
+  double corr_al = 0.0;
+  double *pr_alt = nullptr;
   int divfft_stage = 1;  // fused divergence FFT: stage the divergence rows in LDS first
   int poisson_pq = 13;  // rows per thread of the radial Poisson solve (13 or 26)
   int helm_lpt = 1;  // radial lines per thread in the fused Helmholtz kernel (1 or 2)
@@ -178,6 +192,7 @@ void launch_explicit_qt(tc_ctx *c);
 void launch_explicit_qr(tc_ctx *c);
 void launch_explicit_qz(tc_ctx *c);
 void launch_helmholtz_fused(tc_ctx *c);
+void launch_helmholtz_fused_corr(tc_ctx *c, const double *pr_in, double *pr_out, double al_prev);
 int choose_helm_lpt(const tc_ctx *c);
 void launch_implicit_qt(tc_ctx *c);
 void launch_implicit_qr(tc_ctx *c);
@@ -204,8 +219,9 @@ void launch_triad(tc_ctx *c, int a, int b, int cc, double s);
 // grouped launches over n instances of one shape (c: any instance of the
 // group, for the shape and the tuning); tab: device table of n StepDesc of
 // the substep, dyn/res: device arrays of n
+bool defers_correction(const tc_ctx *c);
 void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
-                              hipStream_t s);
+                              hipStream_t s, bool corr);
 void launch_correct_both_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t s);
 void launch_cfl_check_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, BatchRes *res, int n,
                         hipStream_t s);

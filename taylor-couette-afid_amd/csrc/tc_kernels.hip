@@ -496,7 +496,43 @@ __global__ __launch_bounds__(256) void k_helm_bwd_g(const StepDesc *__restrict__
 // plane k+2 is fetched into registers while plane k is computed.  The
 // explicit terms never reach HBM.
 // ---------------------------------------------------------------------------
+// One point of CorrectVelocity.F90:32-40 and CorrectPressure.F90:37-46
+// (amm = apph), without FMA contraction: the pair kernel and the Helmholtz
+// kernel that applies a deferred correction round the same way.
+// dm: the dph neighbour at i-1 (qt), k-1 (qr), j-1 (qz).
+DEVI double corr_qt(const GridDev &g, int kc, double t, double d, double dm, double dt, double al) {
+#pragma clang fp contract(off)
+  return t - (d - dm) * dt * al * g.dxt * g.usrm[kc];
+}
+DEVI double corr_qr(const GridDev &g, int kc, double r, double d, double dm, double dt, double al) {
+#pragma clang fp contract(off)
+  return r - (d - dm) * dt * al * g.rudx2c[kc];
+}
+DEVI double corr_qz(const GridDev &g, double z, double d, double dm, double dt, double al) {
+#pragma clang fp contract(off)
+  return z - (d - dm) * al * dt * g.dxz;
+}
+DEVI double corr_pr(const GridDev &g, int kc, double p, double d, double dim, double dip, double djm,
+                    double djp, double dkm, double dkp, double be) {
+#pragma clang fp contract(off)
+  const double amm = g.apph[kc], acc = g.acph[kc], app = g.apph[kc];
+  return p + d -
+         be * ((dip - 2.0 * d + dim) * g.udx1qm[kc] + (djp - 2.0 * d + djm) * g.dxzq +
+               (dkp * app + d * acc + dkm * amm));
+}
+
 constexpr int FT_I = 64, FT_W = FT_I + 2;
+
+// A pressure correction still to be applied: the Helmholtz kernel of the
+// next substep applies it to every plane it stages (velocity and pressure)
+// and writes the corrected pressure of its tile to pr_out.  The two arrays
+// travel as separate __restrict__ arguments: through a struct member the
+// compiler could not tell that the pr_out stores leave the grid tables
+// alone, and it then loads those with vector loads whose waits also drain
+// the plane prefetch.
+struct CorrIn {
+  double dt, al, be;  // al of the substep that produced dph
+};
 
 struct LAcc {
   const double *m, *c, *p;  // planes k-1, k, k+1 of one field
@@ -520,7 +556,12 @@ DEVI int xcd_tile(int b, int nb) {
 // LPT lines per thread (rows ty and ty+4).  UNIFORM: every wall mask is 1
 // (no-slip everywhere, the SlipBCs default), so all lines share one table
 // entry and the table pointers stay scalar.
-template <bool UNIFORM, int LPT>
+// CORR: qt, qr, qz hold the previous substep's velocity before its pressure
+// correction and pr its pressure before the correction; the correction
+// (dph of that substep) is applied to each plane as it is staged, from a
+// ring of dph planes with a two-point theta/z halo, so the corrected fields
+// are the ones the per-substep path would have stored, bit for bit.
+template <bool UNIFORM, int LPT, bool CORR>
 DEVI void helm_fwd_body(
     const int tile, const GridDev &g, const ImpArgs &a, const double *__restrict__ qt,
     const double *__restrict__ qr, const double *__restrict__ qz, const double *__restrict__ pr,
@@ -528,10 +569,13 @@ DEVI void helm_fwd_body(
     double *__restrict__ yt, double *__restrict__ yr, double *__restrict__ yz,
     const unsigned char *__restrict__ icyl, const unsigned char *__restrict__ ocyl,
     const double *__restrict__ lut, const double *__restrict__ lur,
-    const double *__restrict__ luz, int nt, int ntx) {
+    const double *__restrict__ luz, int nt, int ntx, const CorrIn &ci,
+    const double *__restrict__ dph, double *__restrict__ pr_out) {
   constexpr int TJ = 4 * LPT, H = TJ + 2, S = FT_W * H, NE = (S + 255) / 256;
+  constexpr int W2 = FT_I + 4, S2 = W2 * (TJ + 4), NE2 = (S2 + 255) / 256;
   __shared__ double sq[3][3][S];  // field, ring slot, tile
   __shared__ double sp[2][S];     // pressure planes k and k+1
+  __shared__ double sd[CORR ? 3 : 1][CORR ? S2 : 1];  // dph planes (slot = plane % 3)
   const int i0 = (tile % ntx) * FT_I, j0 = (tile / ntx) * TJ;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const long long P = g.P;
@@ -549,6 +593,55 @@ DEVI void helm_fwd_body(
     ii = ((ii % g.nthm) + g.nthm) % g.nthm;
     off[q] = (long long)jj * g.nthm + ii;
   }
+  // CORR: the dph tile elements (two-point halo) this thread stages, and
+  // whether each of its velocity tile elements is an interior point whose
+  // corrected pressure it writes
+  int off2[CORR ? NE2 : 1];
+  bool wpr[NE];
+  const bool has_last2 = threadIdx.x + (NE2 - 1) * 256 < S2;
+  if constexpr (CORR) {
+#pragma unroll
+    for (int q = 0; q < NE2; q++) {
+      const int e = threadIdx.x + q * 256;
+      const int ee = e < S2 ? e : 0;
+      int jj = j0 + ee / W2 - 2, ii = i0 + ee % W2 - 2;
+      jj = ((jj % g.nzm) + g.nzm) % g.nzm;
+      ii = ((ii % g.nthm) + g.nthm) % g.nthm;
+      off2[q] = jj * g.nthm + ii;
+    }
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int e = threadIdx.x + q * 256;
+      const int row = e / FT_W, col = e % FT_W;
+      wpr[q] = (e < S) && row >= 1 && row <= TJ && col >= 1 && col <= FT_I &&
+               j0 + row - 1 < g.nzm && i0 + col - 1 < g.nthm;
+    }
+  }
+  // the corrected velocity of plane p at tile element e (correct_pair's
+  // arithmetic; q_r is zero on both walls)
+  auto corr_q = [&](int p, int e, double &vt, double &vr, double &vz) {
+    const int row = e / FT_W, col = e - row * FT_W;
+    const int e2 = (row + 1) * W2 + col + 1;
+    const double *dc = sd[CORR ? p % 3 : 0];
+    if (p <= g.nrm) {
+      vt = corr_qt(g, p, vt, dc[e2], dc[e2 - 1], ci.dt, ci.al);
+      vz = corr_qz(g, vz, dc[e2], dc[e2 - W2], ci.dt, ci.al);
+    }
+    if (p == 1 || p >= g.nr)
+      vr = 0.0;
+    else
+      vr = corr_qr(g, p, vr, dc[e2], sd[CORR ? (p - 1) % 3 : 0][e2], ci.dt, ci.al);
+  };
+  // the corrected pressure of plane p <= nrm at tile element e
+  auto corr_p = [&](int p, int e, double v) {
+    const int row = e / FT_W, col = e - row * FT_W;
+    const int e2 = (row + 1) * W2 + col + 1;
+    const double *dc = sd[CORR ? p % 3 : 0];
+    const double *dkp = sd[CORR ? (p < g.nrm ? p + 1 : p) % 3 : 0];
+    const double *dkm = sd[CORR ? (p > 1 ? p - 1 : p) % 3 : 0];
+    return corr_pr(g, p, v, dc[e2], dc[e2 - 1], dc[e2 + 1], dc[e2 - W2], dc[e2 + W2], dkm[e2],
+                   dkp[e2], ci.be);
+  };
   // per-line state
   bool valid[LPT];
   long long l[LPT];
@@ -575,13 +668,34 @@ DEVI void helm_fwd_body(
     h3[u] = valid[u] ? ru3[l[u]] : 0.0;
     ct[u] = cr[u] = cz[u] = pr_km[u] = 0.0;
   }
-  // planes 1 and 2 of the velocity, plane 1 of the pressure
+  // planes 1 and 2 of the velocity, plane 1 of the pressure (CORR: planes 1
+  // and 2 of dph first, then the corrected values)
+  if constexpr (CORR) {
+    const long long o2 = (g.nrm > 1) ? P : 0;
+#pragma unroll
+    for (int q = 0; q < NE2; q++) {
+      const double a1 = dph[off2[q]], a2 = dph[o2 + off2[q]];
+      if (q < NE2 - 1 || has_last2) {
+        const int e = threadIdx.x + q * 256;
+        sd[1][e] = a1;
+        sd[2][e] = a2;
+      }
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int q = 0; q < NE; q++) {
-    const double v0 = qt[off[q]], v1 = qr[off[q]], v2 = qz[off[q]];
-    const double w0 = qt[P + off[q]], w1 = qr[P + off[q]], w2 = qz[P + off[q]];
-    const double v3 = pr[off[q]];
+    double v0 = qt[off[q]], v1 = qr[off[q]], v2 = qz[off[q]];
+    double w0 = qt[P + off[q]], w1 = qr[P + off[q]], w2 = qz[P + off[q]];
+    double v3 = pr[off[q]];
     if (q < NE - 1 || has_last) {
+      if constexpr (CORR) {
+        const int e = threadIdx.x + q * 256;
+        corr_q(1, e, v0, v1, v2);
+        corr_q(2, e, w0, w1, w2);
+        v3 = corr_p(1, e, v3);
+        if (wpr[q]) pr_out[off[q]] = v3;
+      }
       const int e = threadIdx.x + q * 256;
       sq[0][1][e] = v0;
       sq[1][1][e] = v1;
@@ -600,7 +714,7 @@ DEVI void helm_fwd_body(
     // prefetch plane k+2 of the velocity, plane k+1 of the pressure and of
     // the RK histories.  In the last iteration these re-read the last plane
     // (their values are not used), so the loads carry no condition.
-    double nq[3][NE], np[NE], n1[LPT], n2[LPT], n3[LPT];
+    double nq[3][NE], np[NE], n1[LPT], n2[LPT], n3[LPT], nd[CORR ? NE2 : 1];
     const long long oq = (long long)(k + 2 <= g.nr ? k + 1 : g.nr - 1) * P;
     const long long op = (long long)(k + 1 <= nrm ? k : nrm - 1) * P;
 #pragma unroll
@@ -609,6 +723,12 @@ DEVI void helm_fwd_body(
       nq[1][q] = qr[oq + off[q]];
       nq[2][q] = qz[oq + off[q]];
       np[q] = pr[op + off[q]];
+    }
+    if constexpr (CORR) {
+      // dph plane k+2 (clamped to nrm: beyond it no correction reads dph)
+      const long long od = (long long)(k + 2 <= nrm ? k + 1 : nrm - 1) * P;
+#pragma unroll
+      for (int q = 0; q < NE2; q++) nd[q] = dph[od + off2[q]];
     }
 #pragma unroll
     for (int u = 0; u < LPT; u++) {
@@ -670,6 +790,12 @@ DEVI void helm_fwd_body(
       h2[u] = n2[u];
       h3[u] = n3[u];
     }
+    if constexpr (CORR) {
+      // dph slot (k+2)%3 held plane k-1, last read before the previous barrier
+#pragma unroll
+      for (int q = 0; q < NE2; q++)
+        if (q < NE2 - 1 || has_last2) sd[(k + 2) % 3][threadIdx.x + q * 256] = nd[q];
+    }
     __syncthreads();
     // in the last iteration these fill slots no later plane reads
     const int s2 = (k + 2) % 3;
@@ -677,6 +803,13 @@ DEVI void helm_fwd_body(
     for (int q = 0; q < NE; q++) {
       const int e = threadIdx.x + q * 256;
       if (q < NE - 1 || has_last) {
+        if constexpr (CORR) {
+          corr_q(k + 2, e, nq[0][q], nq[1][q], nq[2][q]);
+          if (k + 1 <= nrm) {
+            np[q] = corr_p(k + 1, e, np[q]);
+            if (wpr[q]) pr_out[(long long)k * P + off[q]] = np[q];
+          }
+        }
         sq[0][s2][e] = nq[0][q];
         sq[1][s2][e] = nq[1][q];
         sq[2][s2][e] = nq[2][q];
@@ -700,21 +833,23 @@ DEVI void helm_fwd_body(
   }
 }
 
-template <bool UNIFORM, int LPT>
+template <bool UNIFORM, int LPT, bool CORR>
 __global__ __launch_bounds__(256) void k_helm_fwd(
     GridDev g, ImpArgs a, const double *__restrict__ qt, const double *__restrict__ qr,
     const double *__restrict__ qz, const double *__restrict__ pr, double *__restrict__ ru1,
     double *__restrict__ ru2, double *__restrict__ ru3, double *__restrict__ yt,
     double *__restrict__ yr, double *__restrict__ yz, const unsigned char *__restrict__ icyl,
     const unsigned char *__restrict__ ocyl, const double *__restrict__ lut,
-    const double *__restrict__ lur, const double *__restrict__ luz, int nt, int ntx, int ntz) {
-  helm_fwd_body<UNIFORM, LPT>(xcd_tile(blockIdx.x, ntx * ntz), g, a, qt, qr, qz, pr, ru1, ru2,
-                              ru3, yt, yr, yz, icyl, ocyl, lut, lur, luz, nt, ntx);
+    const double *__restrict__ lur, const double *__restrict__ luz, int nt, int ntx, int ntz,
+    CorrIn ci, const double *__restrict__ dph, double *__restrict__ pr_out) {
+  helm_fwd_body<UNIFORM, LPT, CORR>(xcd_tile(blockIdx.x, ntx * ntz), g, a, qt, qr, qz, pr, ru1,
+                                    ru2, ru3, yt, yr, yz, icyl, ocyl, lut, lur, luz, nt, ntx, ci,
+                                    dph, pr_out);
 }
 
 // Grouped: blockIdx.x = tile, blockIdx.y = instance of the shape group; the
 // XCD remap runs over the flattened (instance, tile) index.
-template <bool UNIFORM, int LPT>
+template <bool UNIFORM, int LPT, bool CORR>
 __global__ __launch_bounds__(256) void k_helm_fwd_g(const StepDesc *__restrict__ tab,
                                                     const BatchDyn *__restrict__ dyn, int ns,
                                                     int ntx, int ntz) {
@@ -725,9 +860,11 @@ __global__ __launch_bounds__(256) void k_helm_fwd_g(const StepDesc *__restrict__
   ImpArgs a = d.a;
   a.vwall_in = dyn[inst].vwall[ns - 1];
   a.bc1 = dyn[inst].bc1[ns - 1];
-  helm_fwd_body<UNIFORM, LPT>(tile, d.g, a, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.f[TC_PR],
-                              d.f[TC_RU1], d.f[TC_RU2], d.f[TC_RU3], d.f[TC_RHS], d.f[TC_DQ],
-                              d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur, d.luz, d.lu_nt, ntx);
+  const CorrIn ci{d.dt, d.corr_al, d.beta};
+  helm_fwd_body<UNIFORM, LPT, CORR>(tile, d.g, a, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.pr_in,
+                                    d.f[TC_RU1], d.f[TC_RU2], d.f[TC_RU3], d.f[TC_RHS], d.f[TC_DQ],
+                                    d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur, d.luz, d.lu_nt, ntx,
+                                    ci, d.f[TC_DPH], d.pr_out);
 }
 
 static ImpArgs imp_args(const tc_ctx *c) {
@@ -798,14 +935,14 @@ void launch_implicit_qz(tc_ctx *c) {
 
 // Fused path: y_theta -> rhs, y_r -> dq, y_z -> qcap (the explicit terms
 // stay in registers, so those fields serve as scratch).
-template <bool U, int LPT>
-static void launch_fwd(tc_ctx *c) {
+template <bool U, int LPT, bool CORR>
+static void launch_fwd(tc_ctx *c, const double *pr, const CorrIn &ci, double *pr_out) {
   const int ntx = (c->nthm + FT_I - 1) / FT_I, ntz = (c->nzm + 4 * LPT - 1) / (4 * LPT);
-  hipLaunchKernelGGL((k_helm_fwd<U, LPT>), dim3((unsigned)(ntx * ntz)), dim3(256), 0, c->stream,
-                     c->gd, imp_args(c), c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR],
+  hipLaunchKernelGGL((k_helm_fwd<U, LPT, CORR>), dim3((unsigned)(ntx * ntz)), dim3(256), 0,
+                     c->stream, c->gd, imp_args(c), c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], pr,
                      c->f[TC_RU1], c->f[TC_RU2], c->f[TC_RU3], c->f[TC_RHS], c->f[TC_DQ],
                      c->f[TC_QCAP], c->mask, c->mask + c->P, lu_slot(c, LU_QT), lu_slot(c, LU_QR),
-                     lu_slot(c, LU_QZ), c->lu_nt, ntx, ntz);
+                     lu_slot(c, LU_QZ), c->lu_nt, ntx, ntz, ci, c->f[TC_DPH], pr_out);
 }
 
 // Lines per thread of the fused kernel: the variant that needs the fewest
@@ -820,8 +957,8 @@ int choose_helm_lpt(const tc_ctx *c) {
   for (int lpt = 1; lpt <= 2; lpt++) {
     int per_cu = 0;
     hipError_t e = (lpt == 1)
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 1>, 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 2>, 256, 0);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 1, false>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 2, false>, 256, 0);
     if (e != hipSuccess || per_cu <= 0) continue;
     const long long tiles = (long long)((c->nthm + FT_I - 1) / FT_I) * ((c->nzm + 4 * lpt - 1) / (4 * lpt));
     const long long slots = (long long)per_cu * cus;
@@ -838,10 +975,31 @@ int choose_helm_lpt(const tc_ctx *c) {
 
 void launch_helmholtz_fused(tc_ctx *c) {
   timer_begin(c, "helm_fwd");
+  const CorrIn ci{};
   if (c->masks_uniform_one) {
-    if (c->helm_lpt == 2) launch_fwd<true, 2>(c); else launch_fwd<true, 1>(c);
+    if (c->helm_lpt == 2) launch_fwd<true, 2, false>(c, c->f[TC_PR], ci, nullptr);
+    else launch_fwd<true, 1, false>(c, c->f[TC_PR], ci, nullptr);
   } else {
-    if (c->helm_lpt == 2) launch_fwd<false, 2>(c); else launch_fwd<false, 1>(c);
+    if (c->helm_lpt == 2) launch_fwd<false, 2, false>(c, c->f[TC_PR], ci, nullptr);
+    else launch_fwd<false, 1, false>(c, c->f[TC_PR], ci, nullptr);
+  }
+  timer_end(c);
+  launch_bwd(c, 7, "helm_bwd");
+}
+
+// The same with the previous substep's pressure correction (dph, al_prev)
+// applied while staging: the velocity fields hold that substep's uncorrected
+// velocity, pr_in its uncorrected pressure; the corrected pressure goes to
+// pr_out (a different array: neighbouring tiles still read pr_in).
+void launch_helmholtz_fused_corr(tc_ctx *c, const double *pr_in, double *pr_out, double al_prev) {
+  timer_begin(c, "helm_fwd_corr");
+  const CorrIn ci{c->dt, al_prev, c->beta};
+  if (c->masks_uniform_one) {
+    if (c->helm_lpt == 2) launch_fwd<true, 2, true>(c, pr_in, ci, pr_out);
+    else launch_fwd<true, 1, true>(c, pr_in, ci, pr_out);
+  } else {
+    if (c->helm_lpt == 2) launch_fwd<false, 2, true>(c, pr_in, ci, pr_out);
+    else launch_fwd<false, 1, true>(c, pr_in, ci, pr_out);
   }
   timer_end(c);
   launch_bwd(c, 7, "helm_bwd");
@@ -966,19 +1124,12 @@ DEVI void correct_pair(const GridDev &g, double *__restrict__ qt, double *__rest
   const double dim_ = dph[IX(g, kc, j, im)], dip = dph[IX(g, kc, j, ip)];
   const double2 t = ld2(qt, o), r = ld2(qr, o), z = ld2(qz, o), p = ld2(pr, o);
   // CorrectVelocity.F90:32-40
-  st2(qt, o, t.x - (d.x - dim_) * dt * al * g.dxt * g.usrm[kc],
-      t.y - (d.y - d.x) * dt * al * g.dxt * g.usrm[kc]);
-  st2(qr, o, r.x - (d.x - dkm.x) * dt * al * g.rudx2c[kc], r.y - (d.y - dkm.y) * dt * al * g.rudx2c[kc]);
-  st2(qz, o, z.x - (d.x - djm.x) * al * dt * g.dxz, z.y - (d.y - djm.y) * al * dt * g.dxz);
-  // CorrectPressure.F90:37-46 (amm = apph, be = beta)
-  const double amm = g.apph[kc], acc = g.acph[kc], app = g.apph[kc];
-  const double px = p.x + d.x -
-                    be * ((d.y - 2.0 * d.x + dim_) * g.udx1qm[kc] + (djp.x - 2.0 * d.x + djm.x) * g.dxzq +
-                          (dkp.x * app + d.x * acc + dkm.x * amm));
-  const double py = p.y + d.y -
-                    be * ((dip - 2.0 * d.y + d.x) * g.udx1qm[kc] + (djp.y - 2.0 * d.y + djm.y) * g.dxzq +
-                          (dkp.y * app + d.y * acc + dkm.y * amm));
-  st2(pr, o, px, py);
+  st2(qt, o, corr_qt(g, kc, t.x, d.x, dim_, dt, al), corr_qt(g, kc, t.y, d.y, d.x, dt, al));
+  st2(qr, o, corr_qr(g, kc, r.x, d.x, dkm.x, dt, al), corr_qr(g, kc, r.y, d.y, dkm.y, dt, al));
+  st2(qz, o, corr_qz(g, z.x, d.x, djm.x, dt, al), corr_qz(g, z.y, d.y, djm.y, dt, al));
+  // CorrectPressure.F90:37-46 (be = beta)
+  st2(pr, o, corr_pr(g, kc, p.x, d.x, dim_, d.y, djm.x, djp.x, dkm.x, dkp.x, be),
+      corr_pr(g, kc, p.y, d.y, d.x, dip, djm.y, djp.y, dkm.y, dkp.y, be));
   // CorrectVelocity.F90:46-47
   if (kc == 1) st2(qr, IX(g, 1, j, i), 0.0, 0.0);
   if (kc == g.nrm) st2(qr, IX(g, g.nr, j, i), 0.0, 0.0);
@@ -1438,20 +1589,24 @@ void launch_triad(tc_ctx *c, int a, int b, int cc, double s) {
 
 template <bool U, int LPT>
 static void launch_fwd_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
-                         hipStream_t st) {
+                         hipStream_t st, bool corr) {
   const int ntx = (c->nthm + FT_I - 1) / FT_I, ntz = (c->nzm + 4 * LPT - 1) / (4 * LPT);
-  hipLaunchKernelGGL((k_helm_fwd_g<U, LPT>), dim3((unsigned)(ntx * ntz), (unsigned)n), dim3(256), 0,
-                     st, tab, dyn, ns, ntx, ntz);
+  const dim3 grid((unsigned)(ntx * ntz), (unsigned)n);
+  if (corr)
+    hipLaunchKernelGGL((k_helm_fwd_g<U, LPT, true>), grid, dim3(256), 0, st, tab, dyn, ns, ntx, ntz);
+  else
+    hipLaunchKernelGGL((k_helm_fwd_g<U, LPT, false>), grid, dim3(256), 0, st, tab, dyn, ns, ntx, ntz);
 }
 
+// corr: every instance's descriptor carries a deferred correction
 void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
-                              hipStream_t st) {
+                              hipStream_t st, bool corr) {
   if (c->masks_uniform_one) {
-    if (c->helm_lpt == 2) launch_fwd_g<true, 2>(c, tab, dyn, ns, n, st);
-    else launch_fwd_g<true, 1>(c, tab, dyn, ns, n, st);
+    if (c->helm_lpt == 2) launch_fwd_g<true, 2>(c, tab, dyn, ns, n, st, corr);
+    else launch_fwd_g<true, 1>(c, tab, dyn, ns, n, st, corr);
   } else {
-    if (c->helm_lpt == 2) launch_fwd_g<false, 2>(c, tab, dyn, ns, n, st);
-    else launch_fwd_g<false, 1>(c, tab, dyn, ns, n, st);
+    if (c->helm_lpt == 2) launch_fwd_g<false, 2>(c, tab, dyn, ns, n, st, corr);
+    else launch_fwd_g<false, 1>(c, tab, dyn, ns, n, st, corr);
   }
   hipLaunchKernelGGL(k_helm_bwd_g, dim3(line_blocks(c), (unsigned)n), dim3(256), 0, st, tab);
 }

@@ -463,7 +463,8 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
     const char *e = std::getenv("TC_DEFER_CORRECTION");
     c->defer_corr = !(e && std::atoi(e) == 0);
   }
-  if (c->defer_corr) {
+  // the spare pressure array, only where the deferred correction applies
+  if (c->defer_corr && c->p.nsst == 3 && c->nthm % 2 == 0) {
     TRY(dev_alloc(c, (void **)&c->pr_alt, sizeof(double) * (size_t)c->N));
     HTRY(hipMemsetAsync(c->pr_alt, 0, sizeof(double) * (size_t)c->N, c->stream));
   }

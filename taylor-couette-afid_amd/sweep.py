@@ -31,6 +31,7 @@ time to result (arrival to final step) is recorded.
 """
 import concurrent.futures as cf
 import math
+import os
 import threading
 import time
 
@@ -147,11 +148,15 @@ def start_fields(tc, p, seed, amp=1e-2, cfl=0.8):
 
 
 def footprint_bytes(p):
-    """Device bytes of one instance: eleven fields of nr planes, the
-    spectrum, and the small per-instance tables."""
+    """Device bytes of one instance: eleven fields of nr planes (twelve where
+    the RK3 step defers its pressure correction: even nthm, unless
+    TC_DEFER_CORRECTION=0), the spectrum, and the small per-instance tables."""
     nthm, nzm, nrm = p["nthm"], p["nzm"], p["nrm"]
     sstr = ((nthm // 2 + 1) + 7) // 8 * 8
-    return 8 * 11 * (nrm + 1) * nzm * nthm + 16 * nrm * nzm * sstr + (8 << 20)
+    defer = (nthm % 2 == 0 and p.get("nsst", 3) == 3
+             and os.environ.get("TC_DEFER_CORRECTION", "1") != "0")
+    nf = 12 if defer else 11
+    return 8 * nf * (nrm + 1) * nzm * nthm + 16 * nrm * nzm * sstr + (8 << 20)
 
 
 def load_and_project(solver, fields):

@@ -71,7 +71,8 @@ def kernel_bytes(p):
         "correct_both": 9 * R,       # dph qt qr qz pr | qt qr qz pr
         "correct_velocity": 7 * R, "correct_pressure": 3 * R,
         "max_cfl": 3 * R, "max_div": 3 * R,
-        "helm_fwd": 13 * R,          # qt qr qz pr ru1 ru2 ru3 | ru1 ru2 ru3 yt yr yz
+        # RK3 substep 1: ro = 0, the history ru is written but not read
+        "helm_fwd": 10 * R,          # qt qr qz pr | ru1 ru2 ru3 yt yr yz
         # substeps 2 and 3: the previous substep's correction applied on the fly
         "helm_fwd_corr": 15 * R,     # qt qr qz pr dph ru1 ru2 ru3 | ru1 ru2 ru3 yt yr yz pr
         "helm_bwd": 6 * R,           # yt yr yz (U q + y) | qt qr qz

@@ -642,6 +642,9 @@ DEVI void helm_fwd_body(
     return corr_pr(g, p, v, dc[e2], dc[e2 - 1], dc[e2 + 1], dc[e2 - W2], dc[e2 + W2], dkm[e2],
                    dkp[e2], ci.be);
   };
+  // the RK history ru enters as ro*ru only: with ro = 0 (the first RK3
+  // substep) it is not read at all
+  const bool hist = CORR || a.ro != 0.0;
   // per-line state
   bool valid[LPT];
   long long l[LPT];
@@ -663,9 +666,9 @@ DEVI void helm_fwd_body(
     const int v = UNIFORM ? 3 : ic + 2 * oc;
     Tt[u] = lu_view(lut + (size_t)v * nt, nt);
     Tz[u] = lu_view(luz + (size_t)v * nt, nt);
-    h1[u] = valid[u] ? ru1[l[u]] : 0.0;
+    h1[u] = (valid[u] && hist) ? ru1[l[u]] : 0.0;
     h2[u] = 0.0;
-    h3[u] = valid[u] ? ru3[l[u]] : 0.0;
+    h3[u] = (valid[u] && hist) ? ru3[l[u]] : 0.0;
     ct[u] = cr[u] = cz[u] = pr_km[u] = 0.0;
   }
   // planes 1 and 2 of the velocity, plane 1 of the pressure (CORR: planes 1
@@ -730,11 +733,16 @@ DEVI void helm_fwd_body(
 #pragma unroll
       for (int q = 0; q < NE2; q++) nd[q] = dph[od + off2[q]];
     }
+    if (hist) {
 #pragma unroll
-    for (int u = 0; u < LPT; u++) {
-      n1[u] = ru1[op + l[u]];
-      n2[u] = ru2[op + l[u]];
-      n3[u] = ru3[op + l[u]];
+      for (int u = 0; u < LPT; u++) {
+        n1[u] = ru1[op + l[u]];
+        n2[u] = ru2[op + l[u]];
+        n3[u] = ru3[op + l[u]];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < LPT; u++) n1[u] = n2[u] = n3[u] = 0.0;
     }
     const int sm = (k + 2) % 3, sc = k % 3, sn = (k + 1) % 3;  // planes k-1, k, k+1
     const double *spk = sp[k & 1];

@@ -388,7 +388,9 @@ DEVI void theta_inv_body(const long long pb, const double2 *__restrict__ spec,
     if (v0) out[r0 * n + e] = z.x;
     if (v1) out[r1 * n + e] = z.y;
   };
-  fft_line<ME, false, true>(buf, off, 1, pl, t, T, tw, 1.0, ld, st);
+  // per-butterfly passes: 104 VGPRs (4 waves per SIMD) against 165 for the
+  // batched ones, 354 against 367 us at 512x512x385
+  fft_line<ME, false, true, false>(buf, off, 1, pl, t, T, tw, 1.0, ld, st);
 }
 
 template <int ME>

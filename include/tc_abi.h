@@ -146,7 +146,12 @@ int tc_correct_velocity(tc_ctx *ctx);        /* CorrectVelocity.F90:12-50       
 int tc_correct_pressure(tc_ctx *ctx);        /* CorrectPressure.F90:11-52       */
 
 /* TimeMarcher.F90:12-57: one RK3 (or AB2) step at the current dt,
- * including tc_begin_step.  Uses fused kernels unless tc_set_fused(ctx,0). */
+ * including tc_begin_step.  Uses fused kernels unless tc_set_fused(ctx,0).
+ * The fused RK3 step (even nthm) applies the pressure correction of
+ * substeps 1 and 2 inside the next substep's Helmholtz kernel; the fields
+ * are corrected when the call returns, equal to the substep-by-substep
+ * schedule to rounding (TC_DEFER_CORRECTION=0 at tc_create selects that
+ * schedule). */
 int tc_time_marcher(tc_ctx *ctx);
 int tc_set_fused(tc_ctx *ctx, int fused);
 

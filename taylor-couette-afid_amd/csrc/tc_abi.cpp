@@ -319,6 +319,10 @@ tc_tuning resolve_tuning(const tc_tuning *in) {
 // tiles within fft_z_lds, as wide as that allows (wider tiles = longer row
 // segments)
 std::string apply_plans(tc_ctx *c, const tc_tuning &t) {
+  if (t.poisson_pq != 13 && t.poisson_pq != 26)
+    return "poisson_pq must be 13 or 26 (got " + std::to_string(t.poisson_pq) + ")";
+  if (t.helm_lpt < 0 || t.helm_lpt > 2)
+    return "helm_lpt must be 0, 1 or 2 (got " + std::to_string(t.helm_lpt) + ")";
   tc::FFTPlan pt, pz;
   std::string e = tc::make_fft_plan(c->p.nthm, pt, t.fft_t_lds, 256);
   if (e.empty()) e = tc::make_fft_plan(c->p.nzm, pz, t.fft_z_lds, t.fft_z_nt == 256 ? 256 : 512);

@@ -174,3 +174,15 @@ def test_autotune_schedule_runs_and_matches_default(tc):
     assert a.tuning()["poisson_pq"] == t["poisson_pq"]
     a.close()
     b.close()
+
+
+def test_tuning_values_outside_the_documented_sets_are_rejected(tc):
+    p = _params(tc, 32, 24, 17)
+    for bad in ({"poisson_pq": 7}, {"helm_lpt": 3}):
+        with pytest.raises(tc.TCError):
+            tc.Solver(p, tuning=bad)
+    s = tc.Solver(p)
+    with pytest.raises(tc.TCError):
+        s.set_tuning({"poisson_pq": 20})
+    assert s.tuning()["poisson_pq"] == 13
+    s.close()

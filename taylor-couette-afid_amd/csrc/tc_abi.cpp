@@ -464,8 +464,12 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
     HTRY(hipMemsetAsync(c->f[q], 0, sizeof(double) * (size_t)c->N, c->stream));
   }
   {
+    // 1 (default): defer from 8 Mi grid points up, where the saved passes
+    // pay; below that the fields live in the caches and the correction's
+    // extra instructions cost more than they save.  2: always; 0: never.
     const char *e = std::getenv("TC_DEFER_CORRECTION");
-    c->defer_corr = !(e && std::atoi(e) == 0);
+    const int mode = e ? std::atoi(e) : 1;
+    c->defer_corr = mode == 2 || (mode == 1 && (long long)c->nthm * c->nzm * c->nrm >= (8LL << 20));
   }
   // the spare pressure array, only where the deferred correction applies
   if (c->defer_corr && c->p.nsst == 3 && c->nthm % 2 == 0) {

@@ -158,8 +158,8 @@ struct tc_ctx {
   bool fused = true;
   // fused RK3 step: the pressure correction of substeps 1 and 2 is applied
   // by the next substep's Helmholtz kernel instead of its own pass; pr_alt
-  // receives the corrected pressure of substep 1 (TC_DEFER_CORRECTION=0
-  // at tc_create turns this off)
+  // receives the corrected pressure of substep 1 (TC_DEFER_CORRECTION at
+  // tc_create: 1 from 8 Mi grid points up, 2 always, 0 never)
   bool defer_corr = true;
   double corr_al = 0.0;
   double *pr_alt = nullptr;

@@ -149,12 +149,14 @@ def start_fields(tc, p, seed, amp=1e-2, cfl=0.8):
 
 def footprint_bytes(p):
     """Device bytes of one instance: eleven fields of nr planes (twelve where
-    the RK3 step defers its pressure correction: even nthm, unless
-    TC_DEFER_CORRECTION=0), the spectrum, and the small per-instance tables."""
+    the RK3 step defers its pressure correction: even nthm and, by default,
+    at least 8 Mi grid points; TC_DEFER_CORRECTION=2 always, 0 never), the
+    spectrum, and the small per-instance tables."""
     nthm, nzm, nrm = p["nthm"], p["nzm"], p["nrm"]
     sstr = ((nthm // 2 + 1) + 7) // 8 * 8
+    mode = os.environ.get("TC_DEFER_CORRECTION", "1")
     defer = (nthm % 2 == 0 and p.get("nsst", 3) == 3
-             and os.environ.get("TC_DEFER_CORRECTION", "1") != "0")
+             and (mode == "2" or (mode == "1" and nthm * nzm * nrm >= 8 << 20)))
     nf = 12 if defer else 11
     return 8 * nf * (nrm + 1) * nzm * nthm + 16 * nrm * nzm * sstr + (8 << 20)
 

@@ -62,7 +62,9 @@ def _single(tc, p, st, nsteps):
     return out, diags
 
 
-def test_batch_matches_single_instance_bitwise(tc):
+@pytest.mark.parametrize("defer", ["1", "2"])  # default (off at these sizes), forced
+def test_batch_matches_single_instance_bitwise(tc, monkeypatch, defer):
+    monkeypatch.setenv("TC_DEFER_CORRECTION", defer)
     nsteps = 5
     mix = _mix(tc)
     starts = [_start(tc, p, seed) for p, seed in mix]

@@ -65,12 +65,16 @@ def test_steps_match_oracle_small(tc, oracle, fused):
     _compare(dev, orc, p["nrm"])
 
 
+@pytest.mark.parametrize("defer", ["0", "2"])
 @pytest.mark.parametrize("lpt", ["1", "2"])
 @pytest.mark.parametrize("masks", [False, True])
-def test_fused_variants_match_oracle(tc, oracle, monkeypatch, lpt, masks):
+def test_fused_variants_match_oracle(tc, oracle, monkeypatch, lpt, masks, defer):
     """Both tilings of the fused Helmholtz kernel (one or two radial lines per
-    thread), with uniform and with mixed slip/no-slip wall masks."""
+    thread), with uniform and with mixed slip/no-slip wall masks, with the
+    pressure correction after every substep and deferred into the next
+    substep's Helmholtz kernel."""
     monkeypatch.setenv("TC_HELM_LPT", lpt)
+    monkeypatch.setenv("TC_DEFER_CORRECTION", defer)
     p = tc.configs.scaled(tc.configs.small_64(), 96, 40, 29)
     p.update(sizepert=0.05, omegapert=0.3, virotext=0.2, romeg=0.1)
     if masks:

@@ -30,6 +30,16 @@ DEVI double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b
 // multiply by (s * i), s = +-1
 DEVI double2 cmul_si(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
 
+// Non-temporal access for the spectrum and dph where a kernel streams each
+// element exactly once (a streaming copy runs 5.8% faster with both).
+DEVI double2 ldc_nt(const double2 *p) {
+  return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+DEVI void stc_nt(double2 *p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // LDS slot of element x with one padding slot after every 2^sh elements
 DEVI int pidx(int x, int sh) { return x + (x >> sh); }
 
@@ -308,8 +318,8 @@ DEVI void theta_fwd_body(const long long pb, const double *__restrict__ in, cons
     // A = (Z + conj(Zc))/2, B = (Z - conj(Zc))/(2i)
     const double2 A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
     const double2 B = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
-    if (v0) spec[r0 * sstr + k] = A;
-    if (v1) spec[r1 * sstr + k] = B;
+    if (v0) stc_nt(spec + r0 * sstr + k, A);
+    if (v1) stc_nt(spec + r1 * sstr + k, B);
   }
 }
 
@@ -365,8 +375,8 @@ DEVI void theta_inv_body(const long long pb, const double2 *__restrict__ spec,
 #pragma unroll
   for (int u = 0; u < KR; u++) {
     const int k = min(t + u * T, nth2 - 1);
-    As[u] = spec[ra * sstr + k];
-    Bs[u] = spec[rb * sstr + k];
+    As[u] = ldc_nt(spec + ra * sstr + k);
+    Bs[u] = ldc_nt(spec + rb * sstr + k);
   }
 #pragma unroll
   for (int u = 0; u < KR; u++) {
@@ -385,8 +395,8 @@ DEVI void theta_inv_body(const long long pb, const double2 *__restrict__ spec,
   __syncthreads();
   auto ld = [&](int) { return make_double2(0.0, 0.0); };
   auto st = [&](int e, double2 z) {
-    if (v0) out[r0 * n + e] = z.x;
-    if (v1) out[r1 * n + e] = z.y;
+    if (v0) __builtin_nontemporal_store(z.x, out + r0 * n + e);
+    if (v1) __builtin_nontemporal_store(z.y, out + r1 * n + e);
   };
   // per-butterfly passes: 104 VGPRs (4 waves per SIMD) against 165 for the
   // batched ones, 354 against 367 us at 512x512x385
@@ -431,6 +441,8 @@ DEVI void fft_z_body(const int id, double2 *__restrict__ spec, const FFTPlan &pl
   const int c0 = (id % ntile) * L;
   double2 *col = spec + (long long)(id / ntile) * n * sstr + c0 + c;
   const bool ok = c0 + c < nth2;
+  // plain accesses here: with non-temporal ones the z transform ran 7%
+  // slower (365 against 341 us at 512x512x385)
   auto ld = [&](int z) { return ok ? col[(long long)z * sstr] : make_double2(0.0, 0.0); };
   auto st = [&](int z, double2 v) {
     if (ok) col[(long long)z * sstr] = v;
@@ -527,7 +539,7 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
 #pragma unroll
     for (int i = 0; i < PQ; i++) {
       const int row = min(lo + i, n - 1);
-      d[i] = spec[(long long)row * M + mm];
+      d[i] = ldc_nt(spec + (long long)row * M + mm);
     }
 #pragma unroll
     for (int i = 0; i < PQ; i++)
@@ -653,7 +665,7 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
         else if (i == S - 1) x = xl;
         else
           x = make_double2(d[i].x - ar[i] * x0.x - cr[i] * xl.x, d[i].y - ar[i] * x0.y - cr[i] * xl.y);
-        spec[(long long)(lo + i) * M + m] = x;
+        stc_nt(spec + (long long)(lo + i) * M + m, x);
       }
     }
   }

@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void k_implicit_qz(GridDev g, ImpArgs a,
 template <bool FOLD>
 DEVI void usolve_line(const LUView &T, int n, int first, double *__restrict__ q,
                       const double *__restrict__ y, long long l, long long P) {
-  constexpr int CH = 8;
+  constexpr int CH = 16;
   auto put = [&](long long o, double qv, double x) { st_nt(q + o, FOLD ? x : qv + x); };
   double x1 = y[(long long)(n - 1) * P + l] * T.rd(n - 1);
   if (n - 1 >= first) {

@@ -153,14 +153,63 @@ def gpu_leg(tc, p, seed, amp, steps, warmup):
     return (t1 - t0), times, last, copy_gbs, triad_gbs
 
 
-def cpu_leg(p, seed, amp, max_seconds):
-    """Oracle (CPU restatement) on this host: one projection-free start, then
-    RK3 steps of the same workload until max_seconds of stepping."""
+def physical_cores():
+    """Physical cores (distinct package/core pairs) among the CPUs this
+    process may run on, and the number of those logical CPUs."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        top = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(top + "physical_package_id").read().strip(),
+                       open(top + "core_id").read().strip()))
+        except OSError:
+            cores.add(("cpu", c))
+    return len(cores), len(cpus)
+
+
+def cpu_threads():
+    """OpenMP threads of the CPU baseline: one per physical core this
+    process may use, capped by OMP_NUM_THREADS where the host sets it."""
+    phys, _ = physical_cores()
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return min(phys, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else phys
+
+
+def cpu_leg(p, seed, amp, steps, warmup):
+    """The CPU baseline, run in a child process so that its OpenMP runtime
+    starts with one thread per physical core, bound close (OMP_PROC_BIND):
+    the oracle rebuilt -O3 -march=native for this host, one projected start,
+    `warmup` untimed RK3 steps of the same workload, then `steps` timed
+    ones.  Returns the child's JSON record."""
+    import subprocess
+    import tempfile
+
+    threads = cpu_threads()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as tmp:
+        lib = os.path.join(tmp, "liboracle_native.so")
+        subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native",
+                        f"NATIVE_OUT={lib}"], check=True)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg-child", lib,
+                            "--workload", p["_workload"], "--steps", str(steps),
+                            "--warmup", str(warmup)],
+                           env=env, check=True, stdout=subprocess.PIPE, text=True)
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    rec["threads"] = threads
+    rec["host"] = host_info(rec.pop("triad_gbs"))
+    return rec
+
+
+def cpu_leg_child(lib, workload, steps, warmup):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    O.use_library(lib)
     tc = importlib.import_module("taylor-couette-afid_amd")
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    p = tc.configs.WORKLOADS[workload]()
+    seed, amp = tc.configs.SEEDS[workload], tc.configs.AMPLITUDE[workload]
     orc = O.Oracle(p)
     qt, qr, qz = tc.init_perturbed_couette(p, seed, amp)
     orc.set("qt", qt)
@@ -168,21 +217,26 @@ def cpu_leg(p, seed, amp, max_seconds):
     orc.set("qz", qz)
     del qt, qr, qz
     orc.call("project")
-    n, t_steps = 0, 0.0
-    while True:
+    times = []
+    for n in range(warmup + steps):
         t0 = time.perf_counter()
-        orc.step(n)
-        t_steps += time.perf_counter() - t0
-        n += 1
-        if t_steps >= max_seconds or n >= 50:
-            break
+        code, _ = orc.step(n)
+        if code:
+            raise RuntimeError(f"oracle step {n} ended with code {code}")
+        if n >= warmup:
+            times.append(time.perf_counter() - t0)
+        print(f"[bench] cpu baseline step {n}: {time.perf_counter() - t0:.2f} s",
+              file=sys.stderr, flush=True)
     orc.close()
-    return n, t_steps, threads
+    print(json.dumps({"steps": steps, "warmup": warmup, "mean_s": float(np.mean(times)),
+                      "p50_s": float(np.median(times)),
+                      "triad_gbs": O.stream_triad_gbs(1 << 26, 5)}))
 
 
-def host_info():
-    """CPU model, sockets and logical CPUs from /proc/cpuinfo, and the
-    host's STREAM triad bandwidth on the OpenMP threads."""
+def host_info(triad_gbs):
+    """CPU model, sockets and logical CPUs from /proc/cpuinfo and the
+    physical cores this process may use, beside the host's STREAM triad
+    bandwidth measured on the baseline's OpenMP threads."""
     model, sockets, ncpu = "unknown", set(), 0
     try:
         with open("/proc/cpuinfo") as f:
@@ -196,10 +250,10 @@ def host_info():
                     sockets.add(v.strip())
     except OSError:
         pass
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+    phys, usable = physical_cores()
     return {"model": model, "sockets": max(len(sockets), 1), "logical_cpus": ncpu,
-            "triad_gbs": round(O.stream_triad_gbs(1 << 26, 5), 1)}
+            "usable_cpus": usable, "usable_physical_cores": phys,
+            "triad_gbs": round(triad_gbs, 1)}
 
 
 def _alone_step_ms(tc, p, fields, steps=8, warmup=2, tuning=None):
@@ -446,8 +500,10 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="production_512")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="timed RK3 steps of the CPU baseline (after 2 warm-up steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-leg-child", metavar="LIB", help=argparse.SUPPRESS)
     ap.add_argument("--sweep-steps", type=int, default=30,
                     help="timed steps of the batched 48-instance sweep (0: skip)")
     ap.add_argument("--served-jobs", type=int, default=12,
@@ -455,6 +511,9 @@ def main():
     ap.add_argument("--summarize-profile", metavar="OUT",
                     help="write the profiles/ summaries from rocprofv3 CSV output in OUT and exit")
     a = ap.parse_args()
+    if a.cpu_leg_child:
+        cpu_leg_child(a.cpu_leg_child, a.workload, a.steps, a.warmup)
+        return
     if a.summarize_profile:
         tc = importlib.import_module("taylor-couette-afid_amd")
         summarize_profile(a.summarize_profile, a.workload, tc.configs.WORKLOADS[a.workload]())
@@ -524,13 +583,20 @@ def main():
         out["served"] = served_leg(tc, a.served_jobs)
         print(f"[bench] served {out['served']}", file=sys.stderr, flush=True)
     if not a.no_cpu:
-        n, t, threads = cpu_leg(p, seed, amp, a.cpu_seconds)
-        out["cpu_baseline"] = {"value": round(npts * n / t / 1e6, 4), "unit": "Mpt\u00b7RK3-steps/s",
-                               "cores": threads, "kind": "port",
-                               "sample": f"{n} RK3 step(s) of the same {p['nthm']}x{p['nzm']}x"
-                                         f"{p['nrm']} workload from the same start, oracle "
-                                         f"(C restatement, OpenMP), {t:.1f} s",
-                               "host": host_info()}
+        p["_workload"] = a.workload
+        c = cpu_leg(p, seed, amp, a.cpu_steps, 2)
+        out["cpu_baseline"] = {"value": round(npts / c["mean_s"] / 1e6, 4),
+                               "unit": "Mpt\u00b7RK3-steps/s",
+                               "cores": c["threads"], "kind": "port",
+                               "sample": f"{c['steps']} timed RK3 steps after {c['warmup']} warm-up "
+                                         f"steps of the same {p['nthm']}x{p['nzm']}x{p['nrm']} "
+                                         f"workload from the same start; oracle (C restatement, "
+                                         f"OpenMP, -O3 -march=native), one thread per physical "
+                                         f"core, OMP_PROC_BIND=close",
+                               "mean_s_per_step": round(c["mean_s"], 3),
+                               "p50_s_per_step": round(c["p50_s"], 3),
+                               "value_p50": round(npts / c["p50_s"] / 1e6, 4),
+                               "host": c["host"]}
     print(json.dumps(out))
 
 

@@ -12,6 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_PATH = os.path.join(_HERE, "liboracle.so")
 
 FIELDS = {"qt": 0, "qr": 1, "qz": 2, "pr": 3, "dph": 4, "rhs": 5, "dq": 6,
           "qcap": 7, "ru1": 8, "ru2": 9, "ru3": 10}
@@ -58,12 +59,21 @@ _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 
 
+def use_library(path):
+    """Load the restatement from another build of the same source (the
+    benchmark's -O3 -march=native build); call before any other use."""
+    global _PATH
+    if _LIB is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _PATH = path
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = _PATH
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+            raise RuntimeError(f"{path} is not built (run make -C oracle)")
         L = ctypes.CDLL(path)
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.POINTER(OrParams)]

@@ -1200,11 +1200,13 @@ void or_calc_local_divergence(or_state *s) {
 /* ------------------------------------------------------------------ */
 /* FFT: unnormalised DFT X[k] = sum_n x[n] exp(sign*2*pi*i*n*k/N)      */
 /* (the transform FFTW computes, SolvePressureCorrection.F90:106-113)  */
-/* recursive mixed-radix decimation in time over the prime factors of  */
-/* N, twiddles from a per-length table exp(-2 pi i m/N)                */
+/* Iterative self-sorting (Stockham) mixed-radix transform over the    */
+/* factors of N (4 first, then 2, 3, 5, ...), applied to a batch of B  */
+/* interleaved lines: element t of line b is x[2*(t*B + b)].  Twiddles */
+/* exp(-2 pi i m/N) come from a long double table.                     */
 /* ------------------------------------------------------------------ */
 typedef struct {
-  int n, nf, f[64];
+  int n, nst, rad[64];
   double *tw; /* 2n: cos, -sin of 2 pi m / n */
 } fft_plan;
 
@@ -1221,12 +1223,16 @@ static const fft_plan *fft_get_plan(int n) {
     if (!r && g_nplans < OR_MAX_PLANS) {
       fft_plan *pl = &g_plans[g_nplans];
       pl->n = n;
-      pl->nf = 0;
+      pl->nst = 0;
       int m = n;
+      while (m % 4 == 0) {
+        pl->rad[pl->nst++] = 4;
+        m /= 4;
+      }
       while (m > 1) {
         int p = 2;
         while (m % p) p++;
-        pl->f[pl->nf++] = p;
+        pl->rad[pl->nst++] = p;
         m /= p;
       }
       pl->tw = (double *)malloc(2 * (size_t)n * sizeof(double));
@@ -1243,230 +1249,307 @@ static const fft_plan *fft_get_plan(int n) {
   return r;
 }
 
-/* workspace (doubles) needed by fft_rec below for a plan of length n */
-static size_t fft_ws_size(int n) { return 6 * (size_t)n + 8; }
-
-static void fft_rec(const fft_plan *pl, int level, int n, const double *x, int stride, double *out,
-                    int sign, double *work) {
-  if (n == 1) {
-    out[0] = x[0];
-    out[1] = x[1];
-    return;
-  }
-  const int p = pl->f[level], m = n / p;
-  double *sub = work;                 /* p sub-transforms of length m */
-  double *wk2 = work + 2 * (size_t)n; /* deeper levels */
-  for (int r = 0; r < p; r++)
-    fft_rec(pl, level + 1, m, x + 2 * (size_t)r * stride, stride * p, sub + 2 * (size_t)r * m, sign, wk2);
-  const int scale = pl->n / n;
+/* one radix-R pass: sub-transforms of length Ns become length Ns*R */
+static void fft_pass(const fft_plan *pl, int R, int Ns, const double *x, double *y, int B, int sign) {
+  const int n = pl->n, nq = n / R, step = n / (Ns * R), rstep = n / R;
   const double sg = (sign < 0) ? 1.0 : -1.0; /* table holds exp(-i...) */
-  for (int k = 0; k < n; k++) {
-    double ar = 0.0, ai = 0.0;
-    const int km = k % m;
-    for (int r = 0; r < p; r++) {
-      const int e = (int)(((long long)r * k) % n) * scale;
-      const double wr = pl->tw[2 * e], wi = sg * pl->tw[2 * e + 1];
-      const double *v = sub + 2 * ((size_t)r * m + km);
-      ar += wr * v[0] - wi * v[1];
-      ai += wr * v[1] + wi * v[0];
+  double wr[64], wi[64], vr[64], vi[64], dr[64], di[64];
+  double *big = NULL;
+  double *Wr = wr, *Wi = wi, *Vr = vr, *Vi = vi, *Dr = dr, *Di = di;
+  if (R > 64) {
+    big = (double *)malloc(6 * (size_t)R * sizeof(double));
+    Wr = big; Wi = big + R; Vr = big + 2 * R; Vi = big + 3 * R; Dr = big + 4 * R; Di = big + 5 * R;
+  }
+  for (int j = 0; j < nq; j++) {
+    const int k = j % Ns;
+    const int out0 = (j / Ns) * Ns * R + k;
+    for (int r = 0; r < R; r++) {
+      const int e = (int)(((long long)k * r * step) % n);
+      Wr[r] = pl->tw[2 * e];
+      Wi[r] = sg * pl->tw[2 * e + 1];
     }
-    out[2 * k] = ar;
-    out[2 * k + 1] = ai;
+    for (int b = 0; b < B; b++) {
+      for (int r = 0; r < R; r++) {
+        const double *v = x + 2 * ((size_t)(j + r * nq) * B + b);
+        Vr[r] = Wr[r] * v[0] - Wi[r] * v[1];
+        Vi[r] = Wr[r] * v[1] + Wi[r] * v[0];
+      }
+      if (R == 2) {
+        Dr[0] = Vr[0] + Vr[1]; Di[0] = Vi[0] + Vi[1];
+        Dr[1] = Vr[0] - Vr[1]; Di[1] = Vi[0] - Vi[1];
+      } else if (R == 4) {
+        /* w = exp(sign i pi/2) = sign * i */
+        const double ar = Vr[0] + Vr[2], ai = Vi[0] + Vi[2];
+        const double br = Vr[0] - Vr[2], bi = Vi[0] - Vi[2];
+        const double cr = Vr[1] + Vr[3], ci = Vi[1] + Vi[3];
+        const double er = Vr[1] - Vr[3], ei = Vi[1] - Vi[3];
+        const double s = (sign < 0) ? -1.0 : 1.0; /* w e = s i e */
+        const double wer = -s * ei, wei = s * er;
+        Dr[0] = ar + cr; Di[0] = ai + ci;
+        Dr[1] = br + wer; Di[1] = bi + wei;
+        Dr[2] = ar - cr; Di[2] = ai - ci;
+        Dr[3] = br - wer; Di[3] = bi - wei;
+      } else {
+        for (int q = 0; q < R; q++) {
+          double sr = 0.0, si = 0.0;
+          for (int r = 0; r < R; r++) {
+            const int e = (int)(((long long)r * q) % R) * rstep;
+            const double cr = pl->tw[2 * e], ci = sg * pl->tw[2 * e + 1];
+            sr += cr * Vr[r] - ci * Vi[r];
+            si += cr * Vi[r] + ci * Vr[r];
+          }
+          Dr[q] = sr;
+          Di[q] = si;
+        }
+      }
+      for (int r = 0; r < R; r++) {
+        double *o = y + 2 * ((size_t)(out0 + r * Ns) * B + b);
+        o[0] = Dr[r];
+        o[1] = Di[r];
+      }
+    }
   }
+  free(big);
 }
 
-/* in-place c2c of x (interleaved); ws: fft_ws_size(n) + 2n doubles */
-static void fft_c2c_ws(const fft_plan *pl, double *x, int sign, double *ws) {
-  const int n = pl->n;
-  double *out = ws + fft_ws_size(n);
-  fft_rec(pl, 0, n, x, 1, out, sign, ws);
-  memcpy(x, out, 2 * (size_t)n * sizeof(double));
-}
-
-/* r2c: n reals -> n/2+1 complex; ws: fft_ws_size(n) + 4n doubles */
-static void fft_r2c_ws(const fft_plan *pl, const double *in, double *out, double *ws) {
-  const int n = pl->n;
-  double *x = ws + fft_ws_size(n) + 2 * (size_t)n;
-  for (int q = 0; q < n; q++) {
-    x[2 * q] = in[q];
-    x[2 * q + 1] = 0.0;
+/* in-place c2c of B interleaved lines x; ws holds 2 n B doubles */
+static void fft_c2c_batch(const fft_plan *pl, double *x, int B, int sign, double *ws) {
+  double *a = x, *b = ws;
+  int Ns = 1;
+  for (int s = 0; s < pl->nst; s++) {
+    fft_pass(pl, pl->rad[s], Ns, a, b, B, sign);
+    Ns *= pl->rad[s];
+    double *t = a;
+    a = b;
+    b = t;
   }
-  fft_c2c_ws(pl, x, -1, ws);
-  memcpy(out, x, 2 * (size_t)(n / 2 + 1) * sizeof(double));
+  if (a != x) memcpy(x, a, 2 * (size_t)pl->n * B * sizeof(double));
 }
 
-/* c2r: Hermitian extension of the n/2+1 inputs, backward DFT, real part */
-static void fft_c2r_ws(const fft_plan *pl, const double *in, double *out, double *ws) {
+/* r2c of 2B real lines packed as B complex lines z = x_{2b} + i x_{2b+1}
+   (z interleaved as in fft_c2c_batch); out receives the n/2+1 outputs of
+   real line c (c = 0..2B-1) at out[2*(t*2B + c)] */
+static void fft_r2c_pairs(const fft_plan *pl, double *z, int B, double *out, double *ws) {
   const int n = pl->n, h = n / 2 + 1;
-  double *x = ws + fft_ws_size(n) + 2 * (size_t)n;
-  for (int q = 0; q < h; q++) {
-    x[2 * q] = in[2 * q];
-    x[2 * q + 1] = in[2 * q + 1];
+  fft_c2c_batch(pl, z, B, -1, ws);
+  for (int t = 0; t < h; t++) {
+    const int tm = (n - t) % n;
+    for (int b = 0; b < B; b++) {
+      const double *p = z + 2 * ((size_t)t * B + b), *q = z + 2 * ((size_t)tm * B + b);
+      double *o1 = out + 2 * ((size_t)t * 2 * B + 2 * b), *o2 = o1 + 2;
+      /* X1 = (Z(t) + conj Z(n-t))/2, X2 = (Z(t) - conj Z(n-t))/(2i) */
+      o1[0] = 0.5 * (p[0] + q[0]);
+      o1[1] = 0.5 * (p[1] - q[1]);
+      o2[0] = 0.5 * (p[1] + q[1]);
+      o2[1] = -0.5 * (p[0] - q[0]);
+    }
   }
-  for (int q = h; q < n; q++) {
-    x[2 * q] = in[2 * (n - q)];
-    x[2 * q + 1] = -in[2 * (n - q) + 1];
-  }
-  fft_c2c_ws(pl, x, +1, ws);
-  for (int q = 0; q < n; q++) out[q] = x[2 * q];
 }
 
-static double *fft_ws_alloc(int n) {
-  return (double *)malloc((fft_ws_size(n) + 4 * (size_t)n) * sizeof(double));
+/* c2r of 2B Hermitian half-spectra in (layout of fft_r2c_pairs' out): the
+   two real lines 2b, 2b+1 come back as real and imaginary parts of z */
+static void fft_c2r_pairs(const fft_plan *pl, const double *in, int B, double *z, double *ws) {
+  const int n = pl->n, h = n / 2 + 1;
+  for (int t = 0; t < n; t++) {
+    const int u = (t < h) ? t : n - t;
+    const double cj = (t < h) ? 1.0 : -1.0; /* X(n-t) = conj X(t) */
+    for (int b = 0; b < B; b++) {
+      const double *x1 = in + 2 * ((size_t)u * 2 * B + 2 * b), *x2 = x1 + 2;
+      double *o = z + 2 * ((size_t)t * B + b);
+      /* Z = X1 + i X2 */
+      o[0] = x1[0] - cj * x2[1];
+      o[1] = cj * x1[1] + x2[0];
+    }
+  }
+  fft_c2c_batch(pl, z, B, +1, ws);
 }
 
 void or_fft_c2c(int n, double *x, int sign) {
-  double *ws = fft_ws_alloc(n);
-  fft_c2c_ws(fft_get_plan(n), x, sign, ws);
+  const fft_plan *pl = fft_get_plan(n);
+  double *ws = (double *)malloc(2 * (size_t)n * sizeof(double));
+  fft_c2c_batch(pl, x, 1, sign, ws);
   free(ws);
 }
 
 void or_fft_r2c(int n, const double *in, double *out) {
-  double *ws = fft_ws_alloc(n);
-  fft_r2c_ws(fft_get_plan(n), in, out, ws);
-  free(ws);
+  const fft_plan *pl = fft_get_plan(n);
+  double *z = (double *)calloc(2 * (size_t)n, sizeof(double));
+  double *ws = (double *)malloc(2 * (size_t)n * sizeof(double));
+  double *o = (double *)malloc(4 * (size_t)(n / 2 + 1) * sizeof(double));
+  for (int t = 0; t < n; t++) z[2 * t] = in[t];
+  fft_r2c_pairs(pl, z, 1, o, ws);
+  for (int t = 0; t <= n / 2; t++) {
+    out[2 * t] = o[4 * t];
+    out[2 * t + 1] = o[4 * t + 1];
+  }
+  free(z); free(ws); free(o);
 }
 
 void or_fft_c2r(int n, const double *in, double *out) {
-  double *ws = fft_ws_alloc(n);
-  fft_c2r_ws(fft_get_plan(n), in, out, ws);
-  free(ws);
+  const fft_plan *pl = fft_get_plan(n);
+  const int h = n / 2 + 1;
+  double *hin = (double *)calloc(4 * (size_t)h, sizeof(double));
+  double *z = (double *)malloc(2 * (size_t)n * sizeof(double));
+  double *ws = (double *)malloc(2 * (size_t)n * sizeof(double));
+  for (int t = 0; t < h; t++) {
+    hin[4 * t] = in[2 * t];
+    hin[4 * t + 1] = in[2 * t + 1];
+  }
+  fft_c2r_pairs(pl, hin, 1, z, ws);
+  for (int t = 0; t < n; t++) out[t] = z[2 * t];
+  free(hin); free(z); free(ws);
 }
 
 /* ------------------------------------------------------------------ */
 /* SolvePressureCorrection.F90:100-163                                */
 /* layout of the spectrum: (k=1..nrm, kz=1..nzmh, ktheta=1..nthm)     */
+/* The transforms run on blocks of OR_KB consecutive radial planes,   */
+/* which are contiguous in both the field and the spectrum.           */
 /* ------------------------------------------------------------------ */
+#define OR_KB 16
 void or_solve_pressure_correction(or_state *s) {
   const int nrm = s->nrm, nzm = s->nzm, nthm = s->nthm;
   const int nzmh = nzm / 2 + 1;
+  const int nkb = (nrm + OR_KB - 1) / OR_KB;
   size_t ns = (size_t)nrm * nzmh * nthm;
   double *spec = (double *)malloc(2 * ns * sizeof(double));
   const fft_plan *plz = fft_get_plan(nzm), *plt = fft_get_plan(nthm);
 #define SP(k, j, i) (spec + 2 * ((size_t)((k) - 1) + (size_t)nrm * ((size_t)((j) - 1) + (size_t)nzmh * (size_t)((i) - 1))))
-  /* r2c along z for every (r, theta) (fwd_guruplan_y, :150) */
+  /* r2c along z for every (r, theta) (fwd_guruplan_y, :150): planes k0,
+     k0+1 of one theta index packed as one complex line */
 #pragma omp parallel
   {
-    double *line = (double *)malloc((size_t)nzm * sizeof(double));
-    double *cl = (double *)malloc(2 * (size_t)nzmh * sizeof(double));
-    double *ws = fft_ws_alloc(nzm);
+    const int B = OR_KB / 2;
+    double *z = (double *)malloc(2 * (size_t)nzm * B * sizeof(double));
+    double *ws = (double *)malloc(2 * (size_t)nzm * B * sizeof(double));
+    double *o = (double *)malloc(4 * (size_t)nzmh * B * sizeof(double));
 #pragma omp for schedule(static)
-    for (int i = 1; i <= nthm; i++)
-      for (int k = 1; k <= nrm; k++) {
-        for (int j = 1; j <= nzm; j++) line[j - 1] = DPH(k, j, i);
-        fft_r2c_ws(plz, line, cl, ws);
-        for (int j = 1; j <= nzmh; j++) {
-          SP(k, j, i)[0] = cl[2 * (j - 1)];
-          SP(k, j, i)[1] = cl[2 * (j - 1) + 1];
+    for (int blk = 0; blk < nthm * nkb; blk++) {
+      const int i = blk / nkb + 1, k0 = (blk % nkb) * OR_KB + 1;
+      const int nk = (nrm - k0 + 1 < OR_KB) ? nrm - k0 + 1 : OR_KB;
+      for (int j = 1; j <= nzm; j++)
+        for (int b = 0; b < B; b++) {
+          const int ka = k0 + 2 * b, kb = ka + 1;
+          z[2 * ((size_t)(j - 1) * B + b)] = (ka - k0 < nk) ? DPH(ka, j, i) : 0.0;
+          z[2 * ((size_t)(j - 1) * B + b) + 1] = (kb - k0 < nk) ? DPH(kb, j, i) : 0.0;
         }
-      }
-    free(line);
-    free(cl);
-    free(ws);
+      fft_r2c_pairs(plz, z, B, o, ws);
+      for (int j = 1; j <= nzmh; j++)
+        for (int c = 0; c < nk; c++) {
+          SP(k0 + c, j, i)[0] = o[2 * ((size_t)(j - 1) * 2 * B + c)];
+          SP(k0 + c, j, i)[1] = o[2 * ((size_t)(j - 1) * 2 * B + c) + 1];
+        }
+    }
+    free(z); free(ws); free(o);
   }
   /* c2c forward along theta for every (r, kz) (fwd_guruplan_z, :154), then
-     divide by nthm*nzm (:157) */
+     divide by nthm*nzm (:157); c2c backward (:200) after the solve */
   const double norm = (double)(nthm * nzm);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      /* complex tridiagonal per (kz=j, ktheta=i), :171-192 */
 #pragma omp parallel
-  {
-    double *cl = (double *)malloc(2 * (size_t)nthm * sizeof(double));
-    double *ws = fft_ws_alloc(nthm);
+      {
+        double *amphT = (double *)malloc(2 * (size_t)nrm * sizeof(double));
+        double *apphT = (double *)malloc(2 * (size_t)nrm * sizeof(double));
+        double *acphT = (double *)malloc(2 * (size_t)nrm * sizeof(double));
+        double *appph = (double *)malloc(2 * (size_t)nrm * sizeof(double));
+        double *drhs = (double *)malloc(2 * (size_t)nrm * sizeof(double));
+        int *phpiv = (int *)malloc((size_t)nrm * sizeof(int));
 #pragma omp for schedule(static)
-    for (int j = 1; j <= nzmh; j++)
-      for (int k = 1; k <= nrm; k++) {
-        for (int i = 1; i <= nthm; i++) {
-          cl[2 * (i - 1)] = SP(k, j, i)[0];
-          cl[2 * (i - 1) + 1] = SP(k, j, i)[1];
-        }
-        fft_c2c_ws(plt, cl, -1, ws);
-        for (int i = 1; i <= nthm; i++) {
-          SP(k, j, i)[0] = cl[2 * (i - 1)] / norm;
-          SP(k, j, i)[1] = cl[2 * (i - 1) + 1] / norm;
-        }
+        for (int i = 1; i <= nthm; i++)
+          for (int j = 1; j <= nzmh; j++) {
+            for (int k = 1; k <= nrm; k++) {
+              drhs[2 * (k - 1)] = SP(k, j, i)[0];
+              drhs[2 * (k - 1) + 1] = SP(k, j, i)[1];
+              acphT[2 * (k - 1)] = s->acph[k] - s->ak3[j] - s->ak1[i] * (s->usrm[k] * s->usrm[k]);
+              acphT[2 * (k - 1) + 1] = 0.0;
+            }
+            for (int k = 1; k <= nrm - 1; k++) {
+              amphT[2 * (k - 1)] = s->amph[k + 1];
+              amphT[2 * (k - 1) + 1] = 0.0;
+              apphT[2 * (k - 1)] = s->apph[k];
+              apphT[2 * (k - 1) + 1] = 0.0;
+            }
+            int n = nrm;
+            if (i == 1 && j == 1) {
+              /* singular Neumann mode: phi(nrm) = 0 gauge, rows 1..nrm-1 */
+              n = nrm - 1;
+              drhs[2 * (nrm - 1)] = 0.0;
+              drhs[2 * (nrm - 1) + 1] = 0.0;
+            }
+            or_zgttrf(n, amphT, acphT, apphT, appph, phpiv);
+            or_zgttrs(n, amphT, acphT, apphT, appph, phpiv, drhs);
+            for (int k = 1; k <= nrm; k++) {
+              SP(k, j, i)[0] = drhs[2 * (k - 1)];
+              SP(k, j, i)[1] = drhs[2 * (k - 1) + 1];
+            }
+          }
+        free(amphT); free(apphT); free(acphT); free(appph); free(drhs); free(phpiv);
       }
-    free(cl);
-    free(ws);
-  }
-  /* complex tridiagonal per (kz=j, ktheta=i), :171-192 */
+    }
+    const int sign = pass == 0 ? -1 : +1;
 #pragma omp parallel
-  {
-    double *amphT = (double *)malloc(2 * (size_t)nrm * sizeof(double));
-    double *apphT = (double *)malloc(2 * (size_t)nrm * sizeof(double));
-    double *acphT = (double *)malloc(2 * (size_t)nrm * sizeof(double));
-    double *appph = (double *)malloc(2 * (size_t)nrm * sizeof(double));
-    double *drhs = (double *)malloc(2 * (size_t)nrm * sizeof(double));
-    int *phpiv = (int *)malloc((size_t)nrm * sizeof(int));
+    {
+      double *cl = (double *)malloc(2 * (size_t)nthm * OR_KB * sizeof(double));
+      double *ws = (double *)malloc(2 * (size_t)nthm * OR_KB * sizeof(double));
 #pragma omp for schedule(static)
-    for (int i = 1; i <= nthm; i++)
-      for (int j = 1; j <= nzmh; j++) {
-        for (int k = 1; k <= nrm; k++) {
-          drhs[2 * (k - 1)] = SP(k, j, i)[0];
-          drhs[2 * (k - 1) + 1] = SP(k, j, i)[1];
-          acphT[2 * (k - 1)] = s->acph[k] - s->ak3[j] - s->ak1[i] * (s->usrm[k] * s->usrm[k]);
-          acphT[2 * (k - 1) + 1] = 0.0;
-        }
-        for (int k = 1; k <= nrm - 1; k++) {
-          amphT[2 * (k - 1)] = s->amph[k + 1];
-          amphT[2 * (k - 1) + 1] = 0.0;
-          apphT[2 * (k - 1)] = s->apph[k];
-          apphT[2 * (k - 1) + 1] = 0.0;
-        }
-        int n = nrm;
-        if (i == 1 && j == 1) {
-          /* singular Neumann mode: phi(nrm) = 0 gauge, rows 1..nrm-1 */
-          n = nrm - 1;
-          drhs[2 * (nrm - 1)] = 0.0;
-          drhs[2 * (nrm - 1) + 1] = 0.0;
-        }
-        or_zgttrf(n, amphT, acphT, apphT, appph, phpiv);
-        or_zgttrs(n, amphT, acphT, apphT, appph, phpiv, drhs);
-        for (int k = 1; k <= nrm; k++) {
-          SP(k, j, i)[0] = drhs[2 * (k - 1)];
-          SP(k, j, i)[1] = drhs[2 * (k - 1) + 1];
-        }
+      for (int blk = 0; blk < nzmh * nkb; blk++) {
+        const int j = blk / nkb + 1, k0 = (blk % nkb) * OR_KB + 1;
+        const int nk = (nrm - k0 + 1 < OR_KB) ? nrm - k0 + 1 : OR_KB;
+        for (int i = 1; i <= nthm; i++)
+          for (int c = 0; c < OR_KB; c++) {
+            double *d = cl + 2 * ((size_t)(i - 1) * OR_KB + c);
+            if (c < nk) {
+              d[0] = SP(k0 + c, j, i)[0];
+              d[1] = SP(k0 + c, j, i)[1];
+            } else {
+              d[0] = d[1] = 0.0;
+            }
+          }
+        fft_c2c_batch(plt, cl, OR_KB, sign, ws);
+        for (int i = 1; i <= nthm; i++)
+          for (int c = 0; c < nk; c++) {
+            const double *d = cl + 2 * ((size_t)(i - 1) * OR_KB + c);
+            if (pass == 0) {
+              SP(k0 + c, j, i)[0] = d[0] / norm;
+              SP(k0 + c, j, i)[1] = d[1] / norm;
+            } else {
+              SP(k0 + c, j, i)[0] = d[0];
+              SP(k0 + c, j, i)[1] = d[1];
+            }
+          }
       }
-    free(amphT); free(apphT); free(acphT); free(appph); free(drhs); free(phpiv);
-  }
-  /* c2c backward along theta (:200) */
-#pragma omp parallel
-  {
-    double *cl = (double *)malloc(2 * (size_t)nthm * sizeof(double));
-    double *ws = fft_ws_alloc(nthm);
-#pragma omp for schedule(static)
-    for (int j = 1; j <= nzmh; j++)
-      for (int k = 1; k <= nrm; k++) {
-        for (int i = 1; i <= nthm; i++) {
-          cl[2 * (i - 1)] = SP(k, j, i)[0];
-          cl[2 * (i - 1) + 1] = SP(k, j, i)[1];
-        }
-        fft_c2c_ws(plt, cl, +1, ws);
-        for (int i = 1; i <= nthm; i++) {
-          SP(k, j, i)[0] = cl[2 * (i - 1)];
-          SP(k, j, i)[1] = cl[2 * (i - 1) + 1];
-        }
-      }
-    free(cl);
-    free(ws);
+      free(cl); free(ws);
+    }
   }
   /* c2r along z into dph (:204-207) */
 #pragma omp parallel
   {
-    double *line = (double *)malloc((size_t)nzm * sizeof(double));
-    double *cl = (double *)malloc(2 * (size_t)nzmh * sizeof(double));
-    double *ws = fft_ws_alloc(nzm);
+    const int B = OR_KB / 2;
+    double *hin = (double *)malloc(4 * (size_t)nzmh * B * sizeof(double));
+    double *z = (double *)malloc(2 * (size_t)nzm * B * sizeof(double));
+    double *ws = (double *)malloc(2 * (size_t)nzm * B * sizeof(double));
 #pragma omp for schedule(static)
-    for (int i = 1; i <= nthm; i++)
-      for (int k = 1; k <= nrm; k++) {
-        for (int j = 1; j <= nzmh; j++) {
-          cl[2 * (j - 1)] = SP(k, j, i)[0];
-          cl[2 * (j - 1) + 1] = SP(k, j, i)[1];
+    for (int blk = 0; blk < nthm * nkb; blk++) {
+      const int i = blk / nkb + 1, k0 = (blk % nkb) * OR_KB + 1;
+      const int nk = (nrm - k0 + 1 < OR_KB) ? nrm - k0 + 1 : OR_KB;
+      for (int j = 1; j <= nzmh; j++)
+        for (int c = 0; c < 2 * B; c++) {
+          double *d = hin + 2 * ((size_t)(j - 1) * 2 * B + c);
+          if (c < nk) {
+            d[0] = SP(k0 + c, j, i)[0];
+            d[1] = SP(k0 + c, j, i)[1];
+          } else {
+            d[0] = d[1] = 0.0;
+          }
         }
-        fft_c2r_ws(plz, cl, line, ws);
-        for (int j = 1; j <= nzm; j++) DPH(k, j, i) = line[j - 1];
-      }
-    free(line);
-    free(cl);
-    free(ws);
+      fft_c2r_pairs(plz, hin, B, z, ws);
+      for (int j = 1; j <= nzm; j++)
+        for (int c = 0; c < nk; c++)
+          DPH(k0 + c, j, i) = z[2 * ((size_t)(j - 1) * B + c / 2) + (c & 1)];
+    }
+    free(hin); free(z); free(ws);
   }
 #undef SP
   free(spec);

@@ -58,11 +58,31 @@ def interior(a, nrm, kind="cell"):
     return a if kind == "qr" else a[:nrm]
 
 
-def pressure_modulo_gauge(pr, ref):
-    """pr - ref after removing the theta-z mean profile difference, which the
-    singular (0,0) Poisson mode leaves undetermined in the reference."""
-    d = pr - ref
-    return d - d.mean(axis=(1, 2), keepdims=True)
+def compare_fields(dev, orc, nrm, tol=1e-10):
+    """Relative L2 of qt, qr, qz and the full pressure between a device
+    solver and the oracle; asserts each is within tol.  Both fix the gauge
+    of the singular (0,0) Poisson mode the same way (phi(nrm) = 0 on that
+    mode), so the pressure is compared whole, mean profile included."""
+    out = {}
+    for n, sl in (("qt", slice(0, nrm)), ("qr", slice(0, nrm + 1)), ("qz", slice(0, nrm)),
+                  ("pr", slice(0, nrm))):
+        out[n] = rel_l2(dev.get(n)[sl], orc.get(n)[sl])
+    for n, e in out.items():
+        assert e <= tol, f"{n}: rel L2 {e:.3e} ({out})"
+    return out
+
+
+def start_pair(tc, oracle, p, seed, amp):
+    """The seeded projected start on the device, copied into a fresh oracle."""
+    dev = tc.Solver(p)
+    qt, qr, qz = tc.init_perturbed_couette(p, seed, amp)
+    dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+    del qt, qr, qz
+    dev.project()
+    orc = oracle.Oracle(p)
+    for n in ("qt", "qr", "qz"):
+        orc.set(n, dev.get(n))
+    return dev, orc
 
 
 def write_bou(path, p):

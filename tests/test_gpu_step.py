@@ -1,38 +1,24 @@
 """Whole-step parity: the main.F90 loop on the device against the oracle.
 
 Bar (the workload contract): relative L2 <= 1e-10 on qt, qr, qz and on the
-pressure (modulo the theta-z mean profile that the singular (0,0) Poisson
-mode leaves free in the reference), max divergence below RESID, wall torques
-within 1e-9 relative.
+whole pressure (device and oracle fix the gauge of the singular (0,0)
+Poisson mode alike), max divergence below RESID, wall torques within 1e-9
+relative.
 """
 import numpy as np
 import pytest
 
-from helpers import pressure_modulo_gauge, rel_l2
+from helpers import compare_fields, start_pair
 
 pytestmark = pytest.mark.gpu
 
 
 def _start(tc, oracle, p, seed, amp):
-    dev = tc.Solver(p)
-    qt, qr, qz = tc.init_perturbed_couette(p, seed, amp)
-    dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
-    dev.project()
-    orc = oracle.Oracle(p)
-    for n in ("qt", "qr", "qz"):
-        orc.set(n, dev.get(n))
-    return dev, orc
+    return start_pair(tc, oracle, p, seed, amp)
 
 
 def _compare(dev, orc, nrm, tol=1e-10):
-    out = {}
-    for n, sl in (("qt", slice(0, nrm)), ("qr", slice(0, nrm + 1)), ("qz", slice(0, nrm))):
-        out[n] = rel_l2(dev.get(n)[sl], orc.get(n)[sl])
-    a, b = dev.get("pr")[:nrm], orc.get("pr")[:nrm]
-    out["pr"] = np.linalg.norm(pressure_modulo_gauge(a, b)) / np.linalg.norm(b - b.mean(axis=(1, 2), keepdims=True))
-    for n, e in out.items():
-        assert e <= tol, f"{n}: rel L2 {e:.3e} ({out})"
-    return out
+    return compare_fields(dev, orc, nrm, tol)
 
 
 @pytest.mark.parametrize("fused", [True, False])
@@ -108,15 +94,42 @@ def test_variable_dt_steps_match_oracle(tc, oracle):
     _compare(dev, orc, p["nrm"])
 
 
-def test_small_64_twenty_steps(tc, oracle):
+def test_small_64_full_run(tc, oracle):
+    """The 64x64x65 workload as specified: RK3 at fixed dt 1e-3, NTST = 199,
+    i.e. 200 steps ending with code 555 (main.F90:152,223), through the
+    library's own main loop on the device and step by step in the oracle."""
     p = tc.configs.small_64()
     dev, orc = _start(tc, oracle, p, tc.configs.SEEDS["small_64"], 1e-2)
-    for n in range(20):
-        c1, d1 = dev.step(n)
+    print("small_64: 200 RK3 steps, device main loop vs oracle", flush=True)
+    code, last, d1 = dev.run(0, 1000)
+    assert (code, last) == (555, 199), (code, last)
+    for n in range(200):
         c2, d2 = orc.step(n)
-        assert c1 == c2 == 0
+        assert c2 == (555 if n == 199 else 0), (n, c2)
+    assert d1["time"] == d2["time"] and d1["dt"] == d2["dt"] == 1e-3
     e = _compare(dev, orc, p["nrm"])
     ti, to = dev.CalcTorque()
     ri, ro = orc.torque()
     assert abs(ti / ri - 1) < 1e-9 and abs(to / ro - 1) < 1e-9
-    print("small_64 20 steps", e)
+    assert dev.CheckDivergence() < p["resid"]
+    print("small_64 200 steps", e, flush=True)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_ab2_steps_match_oracle(tc, oracle, fused):
+    """NSST = 1: one Adams-Bashforth 2 substep per step (gamma 3/2, rho -1/2,
+    InitTimeMarchScheme.F90:29-36), through both kernel sets, with the
+    inner cylinder oscillating and variable dt."""
+    p = tc.configs.scaled(tc.configs.production_512(), 40, 32, 29)
+    p.update(nsst=1, ntst=1000, sizepert=0.1, omegapert=0.4, virotext=-0.2, dtmax=2e-3)
+    dev, orc = _start(tc, oracle, p, 23, 1e-2)
+    dev.set_fused(fused)
+    for n in range(8):
+        c1, d1 = dev.step(n)
+        c2, d2 = orc.step(n)
+        assert c1 == c2 == 0
+        assert abs(d1["dt"] / d2["dt"] - 1) < 1e-12
+        assert d1["dmax"] < p["resid"]
+        assert abs(d1["torque_in"] / d2["torque_in"] - 1) < 1e-9
+        assert abs(d1["torque_out"] / d2["torque_out"] - 1) < 1e-9
+    _compare(dev, orc, p["nrm"])

@@ -173,12 +173,23 @@ int tc_step(tc_ctx *ctx, int ntime, double walltime, double *out9);
 
 /* The main loop from ntime0 for at most nsteps iterations of tc_step, ending
  * at the first nonzero error code (main.F90:152-261 without the host
- * between steps).  walltime_start is the wall-clock origin of WALLTIMEMAX
- * (seconds on the same clock as the caller's).  On return *ntime_last is
- * the last iteration run and out9 its diagnostics; the result is that
- * iteration's error code, or a negative ABI error. */
+ * between steps).  walltime_start is the wall-clock time already elapsed
+ * when the call starts (seconds); the elapsed time is read after each
+ * iteration's work, as main.F90:208,220 does, and above WALLTIMEMAX gives
+ * code 334 (555 still wins).  On return *ntime_last is the last iteration
+ * run and out9 its diagnostics; the result is that iteration's error code,
+ * or a negative ABI error. */
 int tc_run_steps(tc_ctx *ctx, int ntime0, int nsteps, double walltime_start, int *ntime_last,
                  double *out9);
+
+/* tc_run_steps that also keeps what the reference writes at every logging
+ * iteration (torquenu.out, vmax.out, the divergence print, main.F90:180-216):
+ * row r of log10 (10 doubles) = ntime, then that iteration's out9.  At most
+ * log_cap rows are stored; *nlog = the number of logging iterations run
+ * (may exceed log_cap).  No counterpart in the reference: its loop writes
+ * these lines itself between steps. */
+int tc_run_steps_logged(tc_ctx *ctx, int ntime0, int nsteps, double walltime_start,
+                        int *ntime_last, double *out9, double *log10, int log_cap, int *nlog);
 
 /* Divergence-free projection of the current velocity with al = dt = 1
  * (CalcLocalDivergence + SolvePressureCorrection + CorrectVelocity), used

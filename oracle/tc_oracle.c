@@ -2156,6 +2156,7 @@ int or_step(or_state *s, int ntime, double walltime, double *o) {
       or_calc_stats(s);
       or_calc_dissipation(s);
     }
+    if (!s->variabletstep) o[0] = o[0] * s->dt; /* main.F90:201 */
   }
   if (s->time > s->p.tmax) errorcode = 333;
   if (walltime > s->p.walltimemax) errorcode = 334;

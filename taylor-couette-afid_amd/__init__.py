@@ -64,6 +64,7 @@ ABI_SYMBOLS = [
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
     "tc_regrid_field", "tc_stream_triad", "tc_kernel_timing_only", "tc_run_steps",
+    "tc_run_steps_logged",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -143,6 +144,8 @@ def lib():
     L.tc_calc_velocity_stats.argtypes = [_V, _D, _D]
     L.tc_step.argtypes = [_V, ctypes.c_int, ctypes.c_double, _D]
     L.tc_run_steps.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I, _D]
+    L.tc_run_steps_logged.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I, _D,
+                                      _D, ctypes.c_int, _I]
     L.tc_init_perturbed_couette.argtypes = [ctypes.POINTER(Params), ctypes.c_ulonglong,
                                             ctypes.c_double, _D, _D, _D]
     L.tc_kernel_timing.argtypes = [_V, ctypes.c_int]
@@ -437,6 +440,20 @@ class Solver:
                                          ctypes.byref(n), _dp(o)))
         return code, n.value, dict(zip(["cfl", "dt", "time", "dmax", "torque_in", "torque_out",
                                         "vmax1", "vmax2", "vmax3"], o))
+
+    def run_logged(self, ntime0, nsteps, walltime=0.0, log_cap=None):
+        """run() that also returns one (ntime, diagnostics) row per logging
+        iteration, as the reference's logs record them."""
+        cap = nsteps if log_cap is None else int(log_cap)
+        o = np.zeros(9)
+        rows = np.zeros((max(cap, 1), 10))
+        n, nl = ctypes.c_int(), ctypes.c_int()
+        code = _check(lib().tc_run_steps_logged(self.h, int(ntime0), int(nsteps), float(walltime),
+                                                ctypes.byref(n), _dp(o), _dp(rows), cap,
+                                                ctypes.byref(nl)))
+        keys = ["cfl", "dt", "time", "dmax", "torque_in", "torque_out", "vmax1", "vmax2", "vmax3"]
+        log = [(int(r[0]), dict(zip(keys, r[1:]))) for r in rows[:min(nl.value, cap)]]
+        return code, n.value, dict(zip(keys, o)), log
 
     # --- instrumentation ---
     def kernel_timing(self, on, only=None):

@@ -1063,13 +1063,14 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
     if (rc) return rc;
     if (v[2] > p.limitvel && v[3] > p.limitvel) errorcode = 266;
     out[0] = std::max(0.00000001, v[0]);
+    c->cfl_cached = out[0];
+    c->cfl_valid = true;
+    if (p.idtv == 0) out[0] = out[0] * c->dt;  // main.F90:201
     out[3] = v[1];
     torque_from_sums(c, v[8], v[9], &out[4], &out[5]);
     out[6] = v[2];
     out[7] = v[3];
     out[8] = v[4];
-    c->cfl_cached = out[0];
-    c->cfl_valid = true;
     if (std::fabs(out[3]) > p.resid) errorcode = 169;
     if (c->time > p.tsta) {  // main.F90:188-190
       rc = run_stats(c, true, true);
@@ -1181,21 +1182,51 @@ int tc_last_dissipation(tc_ctx *c, double *nute) {
   return 0;
 }
 
-int tc_run_steps(tc_ctx *c, int ntime0, int nsteps, double walltime_start, int *ntime_last,
-                 double *o) {
+// main.F90:152-261.  The wall clock is read after each iteration's work,
+// as main.F90:208,220 reads ti(2) after TimeMarcher and the logging pass:
+// WALLTIMEMAX then replaces that iteration's code with 334, except 555,
+// which main.F90 assigns last.
+static int run_steps(tc_ctx *c, int ntime0, int nsteps, double walltime_start, int *ntime_last,
+                     double *o, double *log10, int log_cap, int *nlog) {
   CTXCHK(c);
   if (nsteps < 1) return fail("tc_run_steps needs nsteps >= 1");
+  if (log10 && log_cap < 0) return fail("negative log capacity");
   const auto t0 = std::chrono::steady_clock::now();
-  int code = 0, n = ntime0;
+  const tc_params &p = c->p;
+  int code = 0, n = ntime0, nl = 0;
+  double o9[9] = {0};
   for (int q = 0; q < nsteps; q++) {
     n = ntime0 + q;
+    code = tc_step(c, n, -HUGE_VAL, o9);
+    if (code < 0) break;
     const double wall =
         walltime_start + std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    code = tc_step(c, n, wall, o);
+    if (wall > p.walltimemax && code != 555) code = 334;
+    // the logging iterations of main.F90:180 (tc_step logs on the same rule)
+    if (log10 && (n == 1 || std::fmod(c->time, p.tout) < c->dt)) {
+      if (nl < log_cap) {
+        log10[10 * (size_t)nl] = (double)n;
+        for (int v = 0; v < 9; v++) log10[10 * (size_t)nl + 1 + v] = o9[v];
+      }
+      nl++;
+    }
     if (code != 0) break;
   }
   if (ntime_last) *ntime_last = n;
+  if (o)
+    for (int v = 0; v < 9; v++) o[v] = o9[v];
+  if (nlog) *nlog = nl;
   return code;
+}
+
+int tc_run_steps(tc_ctx *c, int ntime0, int nsteps, double walltime_start, int *ntime_last,
+                 double *o) {
+  return run_steps(c, ntime0, nsteps, walltime_start, ntime_last, o, nullptr, 0, nullptr);
+}
+
+int tc_run_steps_logged(tc_ctx *c, int ntime0, int nsteps, double walltime_start, int *ntime_last,
+                        double *o, double *log10, int log_cap, int *nlog) {
+  return run_steps(c, ntime0, nsteps, walltime_start, ntime_last, o, log10, log_cap, nlog);
 }
 
 int tc_init_perturbed_couette(const tc_params *p, unsigned long long seed, double amp, double *qt,

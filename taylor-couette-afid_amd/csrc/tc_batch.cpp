@@ -109,6 +109,27 @@ void free_groups(tc_batch *b) {
   b->groups.clear();
 }
 
+// out9 of the last step an instance took (its end step once it is done),
+// as tc_step reports it: the fixed-dt CFL (cfl * dt, main.F90:172,201) of
+// that step's logging pass or, for a step without one, of its start; the
+// diagnostics of the last logging pass; dt and the time after the step.
+void set_out9(Inst &I, const BatchRes &r) {
+  const tc_ctx *c = I.c;
+  const int last = I.done ? I.end_step : I.steps - 1;
+  if (last < 0) return;
+  const double dt = c->p.dtmax;
+  if (r.log_ntime >= 0) {
+    I.out9[3] = r.diag[1];
+    tc::ctx_torque(c, r.diag[8], r.diag[9], &I.out9[4], &I.out9[5]);
+    I.out9[6] = r.diag[2];
+    I.out9[7] = r.diag[3];
+    I.out9[8] = r.diag[4];
+  }
+  I.out9[0] = (r.log_ntime == last ? std::max(1.0e-8, r.diag[0]) : r.cfl) * dt;
+  I.out9[1] = dt;
+  I.out9[2] = I.time;
+}
+
 // Wait for all queued batch work and fold the device results into the
 // instances: the first failing step and its code, as the main.F90 loop
 // would have reported them, and the last logging values.
@@ -127,16 +148,6 @@ int harvest(tc_batch *b) {
       if (!I.live || !I.c) continue;
       const BatchRes &r = g.res_host[(size_t)q];
       tc_ctx *c = I.c;
-      if (r.log_ntime >= 0) {
-        I.out9[0] = std::max(1.0e-8, r.diag[0]);
-        I.out9[3] = r.diag[1];
-        tc::ctx_torque(c, r.diag[8], r.diag[9], &I.out9[4], &I.out9[5]);
-        I.out9[6] = r.diag[2];
-        I.out9[7] = r.diag[3];
-        I.out9[8] = r.diag[4];
-      }
-      I.out9[1] = c->p.dtmax;
-      I.out9[2] = I.time;
       if (I.done) continue;
       // earliest step with any exit condition; within that step the codes
       // are assigned in main.F90 order 165, 266/169, 333, 555 and the last wins
@@ -155,7 +166,17 @@ int harvest(tc_batch *b) {
       I.done = true;
       I.code = code;
       I.end_step = first;
+      // a device-detected end stopped the instance after that step; the
+      // host clock kept counting until this harvest
+      if (first == r.fail165 - 1 || first == r.faillog - 1) I.time = r.fail_time;
       b->dirty = true;
+    }
+  }
+  for (Group &g : b->groups) {
+    for (size_t q = 0; q < g.ids.size(); q++) {
+      Inst &I = b->inst[(size_t)g.ids[q]];
+      if (!I.live || !I.c || q >= g.res_host.size()) continue;
+      set_out9(I, g.res_host[q]);
     }
   }
   return 0;
@@ -232,19 +253,21 @@ int rebuild(tc_batch *b) {
     const int nsst = g.rep->p.nsst;
     BCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     BCHK(hipEventCreateWithFlags(&g.join, hipEventDisableTiming));
+    BCHK(hipMalloc((void **)&g.dyn, sizeof(BatchDyn) * n));
+    BCHK(hipMalloc((void **)&g.res, sizeof(BatchRes) * n));
     std::vector<StepDesc> h((size_t)n);
     for (int ns = 1; ns <= nsst; ns++) {
       for (int q = 0; q < n; q++) {
         tc_ctx *c = b->inst[(size_t)g.ids[(size_t)q]].c;
         h[(size_t)q] = make_desc(c, ns);
+        h[(size_t)q].dyn = g.dyn + q;
+        h[(size_t)q].res = g.res + q;
       }
       BCHK(hipMalloc((void **)&g.tab[ns - 1], sizeof(StepDesc) * n));
       BCHK(hipMemcpyAsync(g.tab[ns - 1], h.data(), sizeof(StepDesc) * n, hipMemcpyHostToDevice,
                           b->stream));
       BCHK(hipStreamSynchronize(b->stream));
     }
-    BCHK(hipMalloc((void **)&g.dyn, sizeof(BatchDyn) * n));
-    BCHK(hipMalloc((void **)&g.res, sizeof(BatchRes) * n));
     std::vector<BatchRes> r0((size_t)n);
     for (BatchRes &r : r0) {
       std::memset(&r, 0, sizeof r);
@@ -428,6 +451,7 @@ int tc_batch_advance(tc_batch *b, int nsteps, int use_graph, int sync_every) {
         const int ntime = I.steps;
         const double tnew = I.time + dt;
         d.ntime = ntime;
+        d.tnew = tnew;
         d.log = (ntime == 1 || std::fmod(tnew, p.tout) < dt) ? 1 : 0;
       }
       BCHK(hipMemcpyAsync(g.dyn, dh + g.dyn_off, sizeof(BatchDyn) * g.ids.size(),

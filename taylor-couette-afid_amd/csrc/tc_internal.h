@@ -100,6 +100,9 @@ struct StepDesc {
   double corr_al;
   const double *pr_in;
   double *pr_out;
+  // this instance's step scalars and device results (its exit state)
+  const struct BatchDyn *dyn;
+  const struct BatchRes *res;
 };
 
 // Per-instance scalars that change every step (written by the host before
@@ -107,6 +110,7 @@ struct StepDesc {
 // logs, and the step index.
 struct BatchDyn {
   double vwall[3], bc1[3];
+  double tnew;  // time at the end of this step
   int log, ntime;
 };
 
@@ -118,7 +122,21 @@ struct BatchRes {
   int log_ntime;       // ntime of the last logging pass (-1: none)
   int faillog;         // 1 + ntime of the first logging step with code 169 or 266 (0: none)
   int faillog_code;    // that code
+  double fail_time;    // time at the end of the first failing step
 };
+
+// An instance whose run has ended on the device (code 165 at the start of a
+// step, or 169/266 in its logging pass) completes that step, as the main
+// loop does, and takes no further step: every grouped kernel leaves at once
+// for it from the next step on.
+#ifdef __HIPCC__
+__device__ __forceinline__ bool inst_stopped(const StepDesc &d) {
+  if (!d.res) return false;
+  const int a = d.res->fail165, b = d.res->faillog;
+  const int f = (a && b) ? (a < b ? a : b) : (a ? a : b);
+  return f != 0 && d.dyn->ntime >= f;
+}
+#endif
 
 }  // namespace tc
 

@@ -497,6 +497,7 @@ __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict_
 // Grouped fused backward sweep: blockIdx.y = instance; y' = rhs, dq, qcap.
 __global__ __launch_bounds__(256) void k_helm_bwd_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   helm_bwd_body<true>((long long)blockIdx.x * blockDim.x + threadIdx.x, d.g, d.f[TC_QT], d.f[TC_QR],
                 d.f[TC_QZ], d.f[TC_RHS], d.f[TC_DQ], d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur,
                 d.luz, d.lu_nt, 7);
@@ -879,6 +880,7 @@ __global__ __launch_bounds__(256) void k_helm_fwd_g(const StepDesc *__restrict__
   const int lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int inst = lin / ntiles, tile = lin - inst * ntiles;
   const StepDesc &d = tab[inst];
+  if (inst_stopped(d)) return;
   ImpArgs a = d.a;
   a.vwall_in = dyn[inst].vwall[ns - 1];
   a.bc1 = dyn[inst].bc1[ns - 1];
@@ -1173,6 +1175,7 @@ __global__ __launch_bounds__(256) void k_correct_both2(GridDev g, double *__rest
 __global__ __launch_bounds__(256) void k_correct_both2_g(const StepDesc *__restrict__ tab, int nrm) {
   const int inst = blockIdx.z / nrm, kc = blockIdx.z - inst * nrm + 1;
   const StepDesc &d = tab[inst];
+  if (inst_stopped(d)) return;
   const int i = 2 * (blockIdx.x * 64 + threadIdx.x);
   const int j = blockIdx.y * 4 + threadIdx.y;
   if (i >= d.g.nthm || j >= d.g.nzm) return;
@@ -1184,6 +1187,7 @@ __global__ __launch_bounds__(256) void k_correct_both2_g(const StepDesc *__restr
 __global__ __launch_bounds__(256) void k_correct_both_g(const StepDesc *__restrict__ tab, int nrm) {
   const int inst = blockIdx.z / nrm, kc = blockIdx.z - inst * nrm + 1;
   const StepDesc &d = tab[inst];
+  if (inst_stopped(d)) return;
   const GridDev &g = d.g;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y * 4 + threadIdx.y;
@@ -1323,6 +1327,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const doubl
 // Grouped CalcMaxCFL partials: blockIdx.y = instance.
 __global__ __launch_bounds__(RED_THREADS) void k_pointmax_cfl_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   pointmax_body<0>(blockIdx.x, d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.red);
 }
 
@@ -1411,6 +1416,7 @@ __global__ __launch_bounds__(256) void k_combine_g(const StepDesc *__restrict__ 
                                                    int nv, unsigned opmask) {
   if (dyn && !dyn[blockIdx.y].log) return;
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   combine_body(blockIdx.x, d.red, nb, nv, opmask, d.red_out);
 }
 
@@ -1421,16 +1427,20 @@ __global__ void k_cfl_check_g(const StepDesc *__restrict__ tab, const BatchDyn *
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const StepDesc &d = tab[q];
+  if (inst_stopped(d)) return;
   const double cfl = fmax(1.0e-8, d.red_out[0]);
   res[q].cfl = cfl;
-  if (res[q].fail165 == 0 && cfl * d.dt > d.limitcfl) res[q].fail165 = dyn[q].ntime + 1;
+  if (res[q].fail165 == 0 && cfl * d.dt > d.limitcfl) {
+    res[q].fail165 = dyn[q].ntime + 1;
+    res[q].fail_time = dyn[q].tnew;
+  }
 }
 
 // Keep the logging pass's values of the instances that logged this step.
 __global__ void k_diag_store_g(const StepDesc *__restrict__ tab, const BatchDyn *__restrict__ dyn,
                                BatchRes *__restrict__ res, int n) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n || !dyn[q].log) return;
+  if (q >= n || !dyn[q].log || inst_stopped(tab[q])) return;
   const double *o = tab[q].red_out;
   for (int v = 0; v < 10; v++) res[q].diag[v] = o[v];
   res[q].log_ntime = dyn[q].ntime;
@@ -1442,6 +1452,7 @@ __global__ void k_diag_store_g(const StepDesc *__restrict__ tab, const BatchDyn 
   if (code && res[q].faillog == 0) {
     res[q].faillog = dyn[q].ntime + 1;
     res[q].faillog_code = code;
+    if (res[q].fail165 == 0) res[q].fail_time = dyn[q].tnew;
   }
 }
 
@@ -1546,6 +1557,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_diagnostics_g(const StepDesc *_
                                                               const BatchDyn *__restrict__ dyn) {
   if (!dyn[blockIdx.y].log) return;
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   diagnostics_body(blockIdx.x, d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.icyl, d.ocyl,
                    d.virotint, d.virotext, d.del, d.deln, d.pi, d.red);
 }

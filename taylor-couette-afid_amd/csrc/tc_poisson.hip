@@ -345,6 +345,7 @@ __global__ __launch_bounds__(256) void k_div_fft_theta_fwd(DivIn dv, GridDev g,
 template <int ME>
 __global__ __launch_bounds__(256) void k_div_fft_theta_fwd_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   const DivIn dv{d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.usdt};
   theta_fwd_body<ME, true, true>(blockIdx.x, nullptr, dv, d.g, d.spec, d.pt, d.twt,
                                  (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
@@ -415,6 +416,7 @@ __global__ __launch_bounds__(256) void k_fft_theta_inv(const double2 *__restrict
 template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_inv_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   theta_inv_body<ME>(blockIdx.x, d.spec, d.f[TC_DPH], d.pt, d.twt, (long long)d.g.nrm * d.g.nzm,
                      d.g.nth2, d.g.sstr);
 }
@@ -464,6 +466,7 @@ __global__ __launch_bounds__(NT) void k_fft_z_g(const StepDesc *__restrict__ tab
   const int lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int inst = lin / ntiles, id = lin - inst * ntiles;
   const StepDesc &d = tab[inst];
+  if (inst_stopped(d)) return;
   fft_z_body<ME, NT>(id, d.spec, d.pz, d.twz, d.g.nth2, d.g.sstr, ntile, sg);
 }
 
@@ -684,6 +687,7 @@ __global__ __launch_bounds__(512) void k_poisson_tridiag(GridDev g, double2 *__r
 template <int PQ>
 __global__ __launch_bounds__(512) void k_poisson_tridiag_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
+  if (inst_stopped(d)) return;
   poisson_body<PQ>(blockIdx.x, d.g, d.spec, d.inv_norm);
 }
 

@@ -870,6 +870,7 @@ program tc_host
       call CalcMaxCFL(instcfl)
       call CheckDivergence(dmax)
       call CalcTorque
+      if (.not. variabletstep) instcfl = instcfl * dt  ! main.F90:201
       if (abs(dmax) > resid) errorcode = 169
       if (time > bou%tsta) then  ! main.F90:188-199
         call CalcStats

@@ -132,6 +132,11 @@ def test_same_alone_and_in_the_mix(tc):
 
 
 def test_batch_exit_codes_match_main_loop(tc):
+    """The exit code and step of each instance are those of the main.F90
+    loop, and an instance that fails on the device (165 at a step start,
+    169 in a logging pass) takes no step after the failing one even though
+    the host reads the results only every sync_every steps: its fields, time
+    and diagnostics equal the single-instance loop's at the failing step."""
     base = _params(tc, 32, 24, 33)
     cases = [dict(ntst=3), dict(tmax=0.0045), dict(limitcfl=1e-6), dict(ntst=6, resid=1e-30)]
     st = _start(tc, base, 21)
@@ -144,17 +149,46 @@ def test_batch_exit_codes_match_main_loop(tc):
             ids.append((i, p))
         b.advance(10, graph=True, sync_every=3)
         for i, p in ids:
-            steps, code, _ = b.status(i)
+            steps, code, out9 = b.status(i)
+            got = {f: b.solver(i).get(f) for f in FIELDS}
             s = tc.Solver(p)
             _load(s, st)
             want = None
             for n in range(10):
-                c, _ = s.step(n)
+                c, d = s.step(n)
                 if c:
                     want = (n + 1, c)
                     break
-            s.close()
             assert (steps, code) == want, (p, steps, code, want)
+            for f in FIELDS:
+                assert np.array_equal(got[f], s.get(f)), (p, f)
+            # cfl: the batch's step-start CFL pass and the single instance's
+            # cached logging CFL evaluate the same expression
+            assert abs(out9["cfl"] / d["cfl"] - 1) <= 1e-14, (p, out9["cfl"], d["cfl"])
+            logged = n == 1
+            for k in tc.Batch.STATUS[1:3] + (tc.Batch.STATUS[3:] if logged else []):
+                assert out9[k] == d[k], (p, k, out9[k], d[k])
+            s.close()
+
+
+def test_fixed_dt_logging_cfl_is_scaled_by_dt(tc, oracle):
+    """main.F90:172,201: in fixed-dt mode both the step-start CFL and the
+    logging pass's CFL are reported times dt."""
+    p = _params(tc, 32, 24, 33, tout=1e30)
+    st = _start(tc, p, 5)
+    s = tc.Solver(p)
+    _load(s, st)
+    orc = oracle.Oracle(p)
+    for f, a in st.items():
+        orc.set(f, a)
+    for n in range(3):
+        c, d = s.step(n)
+        c2, d2 = orc.step(n)
+        assert abs(d["cfl"] / d2["cfl"] - 1) < 1e-12, (n, d["cfl"], d2["cfl"])
+        if n == 1:  # the logging step
+            cfl_raw = s.CalcMaxCFL()
+            assert abs(d["cfl"] - cfl_raw * p["dtmax"]) <= 1e-15 * cfl_raw * p["dtmax"]
+    s.close()
 
 
 def test_autotune_schedule_runs_and_matches_default(tc):

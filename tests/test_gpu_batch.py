@@ -85,10 +85,14 @@ def test_batch_matches_single_instance_bitwise(tc, monkeypatch, defer):
                 assert np.array_equal(a, ref[f]), (i, f, np.abs(a - ref[f]).max())
             steps, code, d = b.status(i)
             assert steps == nsteps and code == 0
-            # the last logging step is ntime = 1
+            # the last logging step is ntime = 1; maxima are the same bits,
+            # the torque sums run in another order where the single instance
+            # evaluates them inside its last correction pass
             d1 = diags[1][1]
-            for k in ("dmax", "torque_in", "torque_out", "vmax1", "vmax2", "vmax3"):
+            for k in ("dmax", "vmax1", "vmax2", "vmax3"):
                 assert d[k] == d1[k], (k, d[k], d1[k])
+            for k in ("torque_in", "torque_out"):
+                assert abs(d[k] / d1[k] - 1) < 1e-13, (k, d[k], d1[k])
 
 
 def test_batch_graph_replays_match_eager(tc):
@@ -167,7 +171,10 @@ def test_batch_exit_codes_match_main_loop(tc):
             assert abs(out9["cfl"] / d["cfl"] - 1) <= 1e-14, (p, out9["cfl"], d["cfl"])
             logged = n == 1
             for k in tc.Batch.STATUS[1:3] + (tc.Batch.STATUS[3:] if logged else []):
-                assert out9[k] == d[k], (p, k, out9[k], d[k])
+                if k.startswith("torque"):
+                    assert abs(out9[k] / d[k] - 1) < 1e-13, (p, k, out9[k], d[k])
+                else:
+                    assert out9[k] == d[k], (p, k, out9[k], d[k])
             s.close()
 
 

@@ -77,6 +77,9 @@ def kernel_bytes(p):
         "helm_fwd_corr": 15 * R,     # qt qr qz pr dph ru1 ru2 ru3 | ru1 ru2 ru3 yt yr yz pr
         "helm_bwd": 6 * R,           # yt yr yz (U q + y) | qt qr qz
         "diagnostics": 3 * R,        # qt qr qz -> 10 scalars
+        # last correction of a logging step with the diagnostics on the
+        # corrected fields: qt qr qz pr dph | qt qr qz pr
+        "correct_diag": 9 * R,
     }
 
 

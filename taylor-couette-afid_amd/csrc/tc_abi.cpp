@@ -266,6 +266,8 @@ int run_substep(tc_ctx *c) {
   }
   if (c->fused && defer && c->ns < c->p.nsst) {
     c->corr_al = c->al;  // applied by the next substep's Helmholtz kernel
+  } else if (c->fused && c->diag_in_correction && c->ns == c->p.nsst) {
+    tc::launch_correct_diag(c);
   } else if (c->fused) {
     tc::launch_correct_both(c);
   } else {
@@ -583,7 +585,8 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   }
   // reductions
   // per-block partials of the reductions: one block of 256 per (j, i) column group
-  TRY(dev_alloc(c, (void **)&c->red, sizeof(double) * 16 * (size_t)((c->P + 255) / 256)));
+  c->red_cap = 40 * (size_t)((c->P + 255) / 256);
+  TRY(dev_alloc(c, (void **)&c->red, sizeof(double) * c->red_cap));
   TRY(dev_alloc(c, (void **)&c->red_out, sizeof(double) * 16));
   HTRY(hipHostMalloc((void **)&c->red_host, sizeof(double) * 16, hipHostMallocDefault));
   // statistics accumulators (StatRoutines.F90 InitStats: zero)
@@ -1049,13 +1052,20 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
     if (instcfl > p.limitcfl) errorcode = 165;
   }
   double out[9] = {instcfl, c->dt, 0, 0, 0, 0, 0, 0, 0};
+  const double tnew = c->time + c->dt;
+  const bool logs = (ntime == 1 || std::fmod(tnew, p.tout) < c->dt);
+  // a logging iteration evaluates its diagnostics inside the last
+  // correction pass where that pass is the fused one
+  c->diag_in_correction = logs && c->fused && !c->batch_owned && tc::correct_diag_applies(c);
   rc = tc_time_marcher(c);
+  const bool diag_done = c->diag_in_correction;
+  c->diag_in_correction = false;
   if (rc) return rc;
-  c->time = c->time + c->dt;
+  c->time = tnew;
   out[2] = c->time;
-  if (ntime == 1 || std::fmod(c->time, p.tout) < c->dt) {
+  if (logs) {
     // CalcVelocityStats, CalcMaxCFL, CheckDivergence, CalcTorque in one pass
-    tc::launch_diagnostics(c);
+    if (!diag_done) tc::launch_diagnostics(c);
     rc = launch_status();
     if (rc) return rc;
     double v[10];

@@ -377,6 +377,7 @@ int tc_batch_add(tc_batch *b, const tc_params *p, const tc_tuning *t, int *id) {
   int rc = tc_create_tuned(p, t, &c);
   if (rc) return rc;
   // the batch owns time and dt: set the step-invariant tables once
+  c->batch_owned = true;
   c->dt = p->dtmax;
   c->time = 0.0;
   rc = tc::ctx_begin_step(c);

@@ -167,6 +167,7 @@ struct tc_ctx {
   double2 *tw_t = nullptr, *tw_z = nullptr;
   // reductions
   double *red = nullptr;        // device partials
+  size_t red_cap = 0;           // their capacity in doubles
   double *red_out = nullptr;    // device results (16)
   double *red_host = nullptr;   // pinned copy of red_out
   int red_blocks = 0;
@@ -192,6 +193,10 @@ struct tc_ctx {
   double *dump = nullptr;  // slab extraction, 3 x max(P, nrm*nzm), made at first use
   // CFL of the current velocity, when known (set by the logging pass of tc_step)
   bool cfl_valid = false;
+  // tc_step of a logging iteration: the last substep's correction pass also
+  // evaluates the logging diagnostics (launch_correct_diag)
+  bool diag_in_correction = false;
+  bool batch_owned = false;     // advanced by a tc_batch (its tables hold the field pointers)
   double cfl_cached = 0.0;
   // timing
   bool timing = false;
@@ -227,6 +232,8 @@ void launch_torque(tc_ctx *c);
 void launch_velocity_stats(tc_ctx *c);
 // one pass: cfl, max|div|, vmax(3), rms sums(3), torque sums(2)
 void launch_diagnostics(tc_ctx *c);
+bool correct_diag_applies(const tc_ctx *c);
+void launch_correct_diag(tc_ctx *c);
 void launch_stats(tc_ctx *c, bool stats, bool diss);
 void launch_dump(tc_ctx *c, int kind, int k);
 int regrid_field(tc_ctx *c, int field, const double *old, int ntho, int nzo, int nro, int istro,

@@ -1249,7 +1249,9 @@ constexpr int RED_THREADS = 256;
 template <int NV>
 DEVI void block_reduce(double (&v)[NV], const int *ops /*0 sum, 1 max*/, double *out) {
   __shared__ double sh[NV][RED_THREADS / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // linear thread index: the blocks are 256 x 1 or 64 x 4
+  const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+  const int lane = t & 63, wid = t >> 6;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double x = v[q];
@@ -1260,7 +1262,7 @@ DEVI void block_reduce(double (&v)[NV], const int *ops /*0 sum, 1 max*/, double 
     if (lane == 0) sh[q][wid] = x;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (t == 0) {
 #pragma unroll
     for (int q = 0; q < NV; q++) {
       double x = sh[q][0];
@@ -1293,6 +1295,7 @@ template <int WHICH>
 DEVI void pointmax_body(const int blk, const GridDev &g, const double *__restrict__ qt,
                         const double *__restrict__ qr, const double *__restrict__ qz,
                         double *__restrict__ part) {
+#pragma clang fp contract(off)
   double m[1] = {0.0};
   const Col c = column(g, blk);
   if (c.ok) {
@@ -1504,6 +1507,9 @@ DEVI void diagnostics_body(const int blk, const GridDev &g, const double *__rest
                            const unsigned char *__restrict__ ocyl, double virotint,
                            double virotext, double del, double deln, double pi,
                            double *__restrict__ part) {
+  // no FMA contraction: the per-point values equal those of the fused
+  // correction pass (correct_diag_body) and of the CFL pass bit for bit
+#pragma clang fp contract(off)
   double v[10] = {0.0, 0.0, -100.0, -100.0, -100.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const Col c = column(g, blk);
   if (c.ok) {
@@ -1570,6 +1576,210 @@ void launch_diagnostics(tc_ctx *c) {
                      c->f[TC_QT], c->f[TC_QR], c->f[TC_QZ], c->mask, c->mask + c->P,
                      c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red);
   finish_reduction(c, 10, 0x1Fu);
+  timer_end(c);
+}
+
+// ---------------------------------------------------------------------------
+// End of a logging step in one pass: the last substep's CorrectVelocity +
+// CorrectPressure + qr wall reset (the arithmetic of correct_pair) and the
+// logging diagnostics of main.F90:180-186 (those of diagnostics_body) on the
+// corrected fields, so the diagnostics read nothing from HBM themselves.
+// A thread owns a theta pair (i, i+1) of one z row and marches a segment of
+// r-planes: dph(k-1), dph(k) come from the previous plane and qr(k+1) is
+// loaded once for the two planes that use it.  The corrected neighbours
+// qt(i+2) and qz(j+1) are corrected again where they are needed: qt(i+2)
+// comes from the next lane where it holds it, else from the raw field.  So
+// that every raw read (neighbours, and qr(k+1) across r segments) sees the
+// raw value, the corrected velocity goes to the three spare arrays tout,
+// rout, zout (the caller then swaps the field pointers); pr is corrected in
+// place (no pressure neighbour is read).  Blocks: 64 pairs x 4 rows,
+// gridDim.z = segments of r; per-block partials of the ten diagnostics in
+// part (diagnostics_body order).
+// ---------------------------------------------------------------------------
+DEVI void correct_diag_body(const int bx, const int by, const int seg, const int nseg,
+                            const GridDev &g, const double *__restrict__ qt,
+                            const double *__restrict__ qr, const double *__restrict__ qz,
+                            double *__restrict__ pr, double *__restrict__ tout,
+                            double *__restrict__ rout, double *__restrict__ zout,
+                            const double *__restrict__ dph, double dt, double al, double be,
+                            const unsigned char *__restrict__ icyl,
+                            const unsigned char *__restrict__ ocyl, double virotint,
+                            double virotext, double del, double deln, double pi,
+                            double *__restrict__ part, int blk) {
+#pragma clang fp contract(off)
+  double v[10] = {0.0, 0.0, -100.0, -100.0, -100.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int lane = threadIdx.x;
+  const int i = 2 * (bx * 64 + lane);
+  const int j = by * 4 + threadIdx.y;
+  const bool ok = i < g.nthm && j < g.nzm;
+  // the lane whose pair is the next one along theta holds qt(i+2)
+  const bool next_in_wave = lane < 63 && i + 2 < g.nthm;
+  const int k0 = 1 + (int)(((long long)seg * g.nrm) / nseg);
+  const int k1 = (int)(((long long)(seg + 1) * g.nrm) / nseg);
+  const int ic = ok ? i : 0, jc = ok ? j : 0;
+  const int im = wrap_m(ic, g.nthm), ip = wrap_p(ic + 1, g.nthm);
+  const int jp = wrap_p(jc, g.nzm), jm = wrap_m(jc, g.nzm);
+  const long long l = (long long)jc * g.nthm + ic;
+  const long long lim = (long long)jc * g.nthm + im, lip = (long long)jc * g.nthm + ip;
+  const long long ljm = (long long)jm * g.nthm + ic, ljp = (long long)jp * g.nthm + ic;
+  auto ld2 = [](const double *a, long long x) { return *reinterpret_cast<const double2 *>(a + x); };
+  auto st2 = [](double *a, long long x, double u, double w) {
+    *reinterpret_cast<double2 *>(a + x) = make_double2(u, w);
+  };
+  const long long P = g.P;
+  // planes k0-1 and k0 of dph, the raw qr(k0), the corrected qt(k0-1)
+  double2 dkm = ld2(dph, (long long)((k0 > 1 ? k0 - 1 : 1) - 1) * P + l);
+  double2 d = ld2(dph, (long long)(k0 - 1) * P + l);
+  double2 rraw = ld2(qr, (long long)(k0 - 1) * P + l);
+  double2 tprev = make_double2(0.0, 0.0);
+  if (k0 > 1) {
+    const double2 tr = ld2(qt, (long long)(k0 - 2) * P + l);
+    const double dimm = dph[(long long)(k0 - 2) * P + lim];
+    tprev = make_double2(corr_qt(g, k0 - 1, tr.x, dkm.x, dimm, dt, al),
+                         corr_qt(g, k0 - 1, tr.y, dkm.y, dkm.x, dt, al));
+  }
+  for (int kc = k0; kc <= k1; kc++) {
+    const long long b = (long long)(kc - 1) * P;
+    const int kp = kc < g.nrm ? kc + 1 : g.nrm;
+    const double2 dkp = ld2(dph, (long long)(kp - 1) * P + l);
+    const double2 djm = ld2(dph, b + ljm), djp = ld2(dph, b + ljp);
+    const double dim_ = dph[b + lim], dip = dph[b + lip];
+    const double2 t = ld2(qt, b + l), z = ld2(qz, b + l), p = ld2_nt(pr + b + l);
+    // qr(k+1): the wall plane nr is not corrected and ends at zero
+    const double2 rnext = ld2(qr, b + P + l);
+    const double2 tc = make_double2(corr_qt(g, kc, t.x, d.x, dim_, dt, al),
+                                    corr_qt(g, kc, t.y, d.y, d.x, dt, al));
+    double2 rc = make_double2(corr_qr(g, kc, rraw.x, d.x, dkm.x, dt, al),
+                              corr_qr(g, kc, rraw.y, d.y, dkm.y, dt, al));
+    const double2 zc = make_double2(corr_qz(g, z.x, d.x, djm.x, dt, al),
+                                    corr_qz(g, z.y, d.y, djm.y, dt, al));
+    const double2 pc =
+        make_double2(corr_pr(g, kc, p.x, d.x, dim_, d.y, djm.x, djp.x, dkm.x, dkp.x, be),
+                     corr_pr(g, kc, p.y, d.y, d.x, dip, djm.y, djp.y, dkm.y, dkp.y, be));
+    if (ok) {
+      st2_nt(tout + b + l, tc.x, tc.y);
+      // CorrectVelocity.F90:46-47: qr(1) = qr(nr) = 0
+      if (kc == 1)
+        st2_nt(rout + b + l, 0.0, 0.0);
+      else
+        st2_nt(rout + b + l, rc.x, rc.y);
+      st2_nt(zout + b + l, zc.x, zc.y);
+      st2_nt(pr + b + l, pc.x, pc.y);
+      if (kc == g.nrm) {
+        // the plane k = nr: qr's wall, and the unused plane of qt and qz
+        // carried over as it was
+        const long long bn = (long long)g.nrm * P + l;
+        st2_nt(rout + bn, 0.0, 0.0);
+        const double2 tw = ld2(qt, bn), zw = ld2(qz, bn);
+        st2_nt(tout + bn, tw.x, tw.y);
+        st2_nt(zout + bn, zw.x, zw.y);
+      }
+    }
+    {
+      if (kc == 1) rc = make_double2(0.0, 0.0);
+      // corrected neighbours: qr(k+1) (zero at the wall nr), qt(i+2), qz(j+1)
+      double2 rkp = make_double2(0.0, 0.0);
+      if (kc < g.nrm)
+        rkp = make_double2(corr_qr(g, kc + 1, rnext.x, dkp.x, d.x, dt, al),
+                           corr_qr(g, kc + 1, rnext.y, dkp.y, d.y, dt, al));
+      double tn = __shfl_down(tc.x, 1, 64);
+      if (!next_in_wave) tn = corr_qt(g, kc, qt[b + lip], dip, d.y, dt, al);
+      const double2 zr = ld2(qz, b + ljp);
+      const double2 zn = make_double2(corr_qz(g, zr.x, djp.x, d.x, dt, al),
+                                      corr_qz(g, zr.y, djp.y, d.y, dt, al));
+      if (ok) {
+        const double tt[2] = {tc.x, tc.y}, tip[2] = {tc.y, tn};
+        const double rr[2] = {rc.x, rc.y}, rp[2] = {rkp.x, rkp.y};
+        const double zz[2] = {zc.x, zc.y}, zp[2] = {zn.x, zn.y};
+        const double tpv[2] = {tprev.x, tprev.y};
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const double t_c = tt[e], t_ip = tip[e], r_c = rr[e], r_kp = rp[e], z_c = zz[e],
+                       z_jp = zp[e];
+          const double cfl = (fabs((t_ip + t_c) * 0.5 * g.dxt * g.usrm[kc]) +
+                              fabs((r_kp + r_c) * 0.5 * g.udx2rm[kc]) + fabs((z_jp + z_c) * 0.5 * g.dxz));
+          const double dv =
+              ((t_ip - t_c) * g.dxt + (r_kp - r_c) * g.udx2m[kc]) * g.usrm[kc] + (z_jp - z_c) * g.dxz;
+          v[0] = fmax(v[0], cfl);
+          v[1] = fmax(v[1], fabs(dv));
+          v[2] = fmax(v[2], fabs(t_c));
+          v[3] = fmax(v[3], fabs(r_c));
+          v[4] = fmax(v[4], fabs(z_c));
+          const double tsum = t_c + t_ip;
+          v[5] += tsum * tsum * g.pvol[kc] * pi * 0.25;
+          const double u = r_c * g.usrc[kc] + r_kp * g.usrc[kc + 1];
+          v[6] += u * u * g.pvol[kc] * pi * 0.25;
+          const double w = z_c + z_jp;
+          v[7] += w * w * g.pvol[kc] * pi * 0.25;
+          if (kc == 2) {
+            const double grtlow = ((t_c * g.usrm[2] + tpv[e] * g.usrm[1]) * 0.5 - virotint * g.usrc[1]) * del;
+            v[8] += grtlow * (double)icyl[l + e];
+          }
+          if (kc == g.nrm) {
+            const double grtup =
+                (virotext * g.usrc[g.nr] - (tpv[e] * g.usrm[g.nrm - 1] + t_c * g.usrm[g.nrm]) * 0.5) * deln;
+            v[9] += grtup * (double)ocyl[l + e];
+          }
+        }
+      }
+      tprev = tc;
+    }
+    dkm = d;
+    d = dkp;
+    rraw = rnext;
+  }
+  const int ops[10] = {1, 1, 1, 1, 1, 0, 0, 0, 0, 0};
+  block_reduce<10>(v, ops, part + 10 * blk);
+}
+
+__global__ __launch_bounds__(256) void k_correct_diag(
+    GridDev g, const double *__restrict__ qt, const double *__restrict__ qr,
+    const double *__restrict__ qz, double *__restrict__ pr, double *__restrict__ tout,
+    double *__restrict__ rout, double *__restrict__ zout, const double *__restrict__ dph,
+    double dt, double al, double be, const unsigned char *__restrict__ icyl,
+    const unsigned char *__restrict__ ocyl, double virotint, double virotext, double del,
+    double deln, double pi, double *__restrict__ part) {
+  const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  correct_diag_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, g, qt, qr, qz, pr, tout, rout,
+                    zout, dph, dt, al, be, icyl, ocyl, virotint, virotext, del, deln, pi, part,
+                    blk);
+}
+
+// Segments of r per column: enough blocks to fill the device about four
+// blocks per CU, each segment at least 16 planes, and the partials within
+// the reduction buffer (TC_CDIAG_SEG overrides).
+int correct_diag_segments(const tc_ctx *c) {
+  const long long cols = (long long)((c->nthm / 2 + 63) / 64) * ((c->nzm + 3) / 4);
+  static const int env = [] {
+    const char *e = std::getenv("TC_CDIAG_SEG");
+    return e ? std::atoi(e) : 0;
+  }();
+  long long s = env > 0 ? env : (1024 + cols - 1) / cols;
+  s = std::min<long long>(s, std::max(1, c->nrm / 16));
+  s = std::min<long long>(s, (long long)c->red_cap / (10 * cols));
+  return (int)std::max<long long>(s, 1);
+}
+
+bool correct_diag_applies(const tc_ctx *c) { return c->nthm % 2 == 0 && c->nrm >= 3; }
+
+// The corrected qt, qr, qz land in the spare arrays rhs, dq, qcap (free at
+// the end of a substep), whose pointers then trade places with qt, qr, qz.
+void launch_correct_diag(tc_ctx *c) {
+  const int nseg = correct_diag_segments(c);
+  const dim3 grid((unsigned)((c->nthm / 2 + 63) / 64), (unsigned)((c->nzm + 3) / 4), (unsigned)nseg);
+  const double del = -1.0 / (c->g.rc[2] - c->g.rc[1]);
+  const double deln = -1.0 / (c->g.rc[c->nr] - c->g.rc[c->nrm]);
+  timer_begin(c, "correct_diag");
+  hipLaunchKernelGGL(k_correct_diag, grid, dim3(64, 4), 0, c->stream, c->gd, c->f[TC_QT],
+                     c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR], c->f[TC_RHS], c->f[TC_DQ],
+                     c->f[TC_QCAP], c->f[TC_DPH], c->dt, c->al, c->beta, c->mask, c->mask + c->P,
+                     c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red);
+  std::swap(c->f[TC_QT], c->f[TC_RHS]);
+  std::swap(c->f[TC_QR], c->f[TC_DQ]);
+  std::swap(c->f[TC_QZ], c->f[TC_QCAP]);
+  hipLaunchKernelGGL(k_combine, dim3(10u), dim3(256), 0, c->stream, c->red,
+                     (int)(grid.x * grid.y * grid.z), 10, 0x1Fu, c->red_out);
+  hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * 10, hipMemcpyDeviceToHost, c->stream);
   timer_end(c);
 }
 

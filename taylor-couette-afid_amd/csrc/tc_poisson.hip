@@ -505,27 +505,60 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
   double *gcp = (double *)(gline + n);                          // its multipliers
   double *cam = gcp + n, *cap = cam + n, *cac = cap + n, *cus = cac + n;  // row k at [k-1]
   const bool gblock = (blk == 0);
+  const int lo = (int)(((long long)pp * n) / PT), S = (int)(((long long)(pp + 1) * n) / PT) - lo;
+  double ar[PQ], cr[PQ];
+  double2 d[PQ];
+  // this thread's rows first, as one batch of unconditional loads (clamped
+  // row, last line for idle lanes), so that their latency overlaps the
+  // staging below; rows past the partition and idle lanes are zeroed after
+  // the barrier
+  {
+    const long long mm = m < M ? m : M - 1;
+#pragma unroll
+    for (int i = 0; i < PQ; i++) {
+      const int row = min(lo + i, n - 1);
+      d[i] = ldc_nt(spec + (long long)row * M + mm);
+    }
+  }
   if (gblock)
     for (int k = tid; k < n; k += nthr) {
       const double2 v = spec[(long long)k * M];
       gline[k] = make_double2(v.x * inv_norm, v.y * inv_norm);
     }
-  // the line-independent radial coefficients, staged once per block
-  for (int k = tid; k < n; k += nthr) {
-    cam[k] = g.amph[k + 1];
-    cap[k] = g.apph[k + 1];
-    cac[k] = g.acph[k + 1];
-    cus[k] = g.usrm[k + 1] * g.usrm[k + 1];
+  // this line's wavenumber terms and the line-independent radial
+  // coefficients, staged once per block: all loads issued before any wait
+  const long long mv = m < M ? m : 0;
+  const int kzl = (int)(mv / g.sstr), ktl = (int)(mv - (long long)kzl * g.sstr);
+  const double szl = g.ak3[kzl + 1], stl = g.ak1[min(ktl, g.nth2 - 1) + 1];
+  {
+    double cv[2][4];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int k = min(tid + q * nthr, n - 1);
+      cv[q][0] = g.amph[k + 1];
+      cv[q][1] = g.apph[k + 1];
+      cv[q][2] = g.acph[k + 1];
+      cv[q][3] = g.usrm[k + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int k = tid + q * nthr;
+      if (k < n) {
+        cam[k] = cv[q][0];
+        cap[k] = cv[q][1];
+        cac[k] = cv[q][2];
+        cus[k] = cv[q][3] * cv[q][3];
+      }
+    }
+    for (int k = tid + 2 * nthr; k < n; k += nthr) {
+      cam[k] = g.amph[k + 1];
+      cap[k] = g.apph[k + 1];
+      cac[k] = g.acph[k + 1];
+      cus[k] = g.usrm[k + 1] * g.usrm[k + 1];
+    }
   }
-  bool valid = m < M && m != 0;
-  int kz = 0, kt = 0;
-  if (valid) {
-    kz = (int)(m / g.sstr);
-    kt = (int)(m - (long long)kz * g.sstr);
-    valid = kt < g.nth2;
-  }
-  const double sz = valid ? g.ak3[kz + 1] : 0.0, st = valid ? g.ak1[kt + 1] : 0.0;
-  const int lo = (int)(((long long)pp * n) / PT), S = (int)(((long long)(pp + 1) * n) / PT) - lo;
+  const bool valid = m < M && m != 0 && ktl < g.nth2;
+  const double sz = valid ? szl : 0.0, st = valid ? stl : 0.0;
   // row k (1-based) of this line: a x(k-1) + b x(k) + c x(k+1) = d
   auto coef = [&](int k, double &a, double &b, double &c) {
     a = (k > 1) ? cam[k - 1] : 0.0;
@@ -533,23 +566,11 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
     b = cac[k - 1] - sz - st * cus[k - 1];
     if (!valid) { a = 0.0; b = 1.0; c = 0.0; }
   };
-  double ar[PQ], cr[PQ];
-  double2 d[PQ];
-  // unconditional loads (clamped row, line 0 for idle lanes) issued as one
-  // batch; rows past the partition and idle lanes are zeroed after
-  {
-    const long long mm = valid ? m : 0;
-#pragma unroll
-    for (int i = 0; i < PQ; i++) {
-      const int row = min(lo + i, n - 1);
-      d[i] = ldc_nt(spec + (long long)row * M + mm);
-    }
-#pragma unroll
-    for (int i = 0; i < PQ; i++)
-      d[i] = (i < S && valid) ? make_double2(d[i].x * inv_norm, d[i].y * inv_norm)
-                              : make_double2(0.0, 0.0);
-  }
   __syncthreads();  // coefficient tables staged
+#pragma unroll
+  for (int i = 0; i < PQ; i++)
+    d[i] = (i < S && valid) ? make_double2(d[i].x * inv_norm, d[i].y * inv_norm)
+                            : make_double2(0.0, 0.0);
   // downward: rows i >= 1 become a_i x_0 + x_i + c_i x_(i+1) = d_i
 #pragma unroll
   for (int i = 0; i < PQ; i++) {

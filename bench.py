@@ -281,10 +281,13 @@ def _alone_step_ms(tc, p, fields, steps=8, warmup=2, tuning=None):
     return ms, setup
 
 
-def sweep_leg(tc, steps_timed):
+def sweep_leg(tc, steps_timed, plain_mpts):
     """Batched sweep of 48 uneven instances: per-step time of the grouped,
     graph-replayed step of all instances, against the same jobs run one at a
-    time on the plain path (per-shape step times, whole-job time summed)."""
+    time on the plain path (per-shape step times, whole-job time summed).
+    Both sides are reported with their setup (context creation, upload,
+    projection) and without it, and beside the bound that the plain
+    512x512x385 rate (plain_mpts) puts on any batching of this job mix."""
     from importlib import import_module
     sw = import_module("taylor-couette-afid_amd.sweep")
     jobs = sw.sweep_jobs()
@@ -304,10 +307,15 @@ def sweep_leg(tc, steps_timed):
     for sh in shapes:
         p, f = next((q, f) for q, f in prepared if (q["nthm"], q["nzm"], q["nrm"]) == sh)
         alone[sh] = _alone_step_ms(tc, p, f)
+    t0 = time.perf_counter()
     b = tc.Batch()
     for p, f in prepared:
         iid = b.add(p, tuning[(p["nthm"], p["nzm"], p["nrm"])])
         sw.load_and_project(b.solver(iid), f)
+    b.groups()
+    for i in range(len(jobs)):
+        b.solver(i).synchronize()
+    setup_batch = time.perf_counter() - t0
     del prepared
     ngroups, launches = b.groups()
     b.advance(2, graph=True, sync_every=2)
@@ -321,69 +329,68 @@ def sweep_leg(tc, steps_timed):
     cells = sum(p["nthm"] * p["nzm"] * p["nrm"] for p, _ in jobs)
     steps_job = jobs[0][0]["ntst"] + 1
     ms = 1e3 * el / steps_timed
-    jobs_h = len(jobs) / (steps_job * ms / 1e3) * 3600
-    alone_s = sum(alone[(p["nthm"], p["nzm"], p["nrm"])][0] * steps_job / 1e3 +
-                  alone[(p["nthm"], p["nzm"], p["nrm"])][1] for p, _ in jobs)
-    alone_jobs_h = len(jobs) / alone_s * 3600
+    key = lambda p: (p["nthm"], p["nzm"], p["nrm"])
+    alone_steps_s = sum(alone[key(p)][0] * steps_job / 1e3 for p, _ in jobs)
+    alone_setup_s = sum(alone[key(p)][1] for p, _ in jobs)
+    batch_steps_s = steps_job * ms / 1e3
+    with_setup = (alone_steps_s + alone_setup_s) / (batch_steps_s + setup_batch)
+    steps_only = alone_steps_s / batch_steps_s
+    bound_s = cells * steps_job / (plain_mpts * 1e6)
     return {
         "workload": "48 uneven FP64 instances (64^2x65, 128^2x97, 192x128x129, 256^2x193), "
                     "eta, 1/NU, VIROTEXT/VIROTINT, LAMB seeded; fixed dt at CFL 0.8; 300-step jobs",
         "instances": len(jobs), "groups": ngroups, "launches_per_step": launches,
         "steps_timed": steps_timed, "ms_per_step": round(ms, 3),
         "mpt_steps_per_s": round(cells * steps_timed / el / 1e6, 1),
-        "jobs_per_h": round(jobs_h, 1),
-        "one_at_a_time_jobs_per_h": round(alone_jobs_h, 1),
-        "speedup_vs_one_at_a_time": round(jobs_h / alone_jobs_h, 2),
+        "jobs_per_h": round(len(jobs) / (batch_steps_s + setup_batch) * 3600, 1),
+        "one_at_a_time_jobs_per_h": round(len(jobs) / (alone_steps_s + alone_setup_s) * 3600, 1),
+        "speedup_vs_one_at_a_time": round(with_setup, 2),
+        "speedup_steps_only": round(steps_only, 2),
+        "setup_s": {"batched": round(setup_batch, 2), "one_at_a_time": round(alone_setup_s, 2)},
+        "speedup_bound_at_plain_512_rate": round(alone_steps_s / bound_s, 2),
         "one_at_a_time_ms_per_step": {f"{a}x{b}x{c}": round(v[0], 3) for (a, b, c), v in alone.items()},
     }
 
 
 def served_leg(tc, njobs):
     """Served sweep: Poisson arrivals (2 jobs/s) of 1000-step jobs through the
-    continuous-batching server, against a FIFO of the same jobs run one at a
-    time on the plain path (service time = measured setup + 1000 x measured
-    step time of the job's shape)."""
+    continuous-batching server, against the same arrival trace served one
+    job at a time on the plain path (run, not modelled)."""
     from importlib import import_module
     sw = import_module("taylor-couette-afid_amd.sweep")
     jobs = sw.served_jobs(njobs)
     shapes = sorted({(p["nthm"], p["nzm"], p["nrm"]) for _, p, _ in jobs})
-    tuning, alone = {}, {}
+    tuning = {}
     for sh in shapes:
         _, p, seed = next(j for j in jobs if (j[1]["nthm"], j[1]["nzm"], j[1]["nrm"]) == sh)
         p = dict(p)
-        f = sw.start_fields(tc, p, seed)
+        sw.start_fields(tc, p, seed)
         tuning[sh] = tc.autotune(p, steps=3)
-        alone[sh] = _alone_step_ms(tc, p, f)
-        del f
+
+    def summary(rec):
+        codes = [r["code"] for r in rec]
+        if any(c != 555 for c in codes):
+            raise RuntimeError(f"served jobs ended with {codes}")
+        ttr = np.array([r["done"] - r["arrival"] for r in rec])
+        makespan = max(r["done"] for r in rec)
+        cell_steps = sum(r["cells"] * r["steps"] for r in rec)
+        return {"p50_s": round(float(np.percentile(ttr, 50)), 2),
+                "p99_s": round(float(np.percentile(ttr, 99)), 2),
+                "jobs_per_h": round(njobs / makespan * 3600, 1),
+                "mpt_steps_per_s": round(cell_steps / makespan / 1e6, 1)}, makespan
+
     srv = sw.Server(tc, tuning=tuning)
-    rec = srv.run(jobs)
-    codes = [r["code"] for r in rec]
-    if any(c != 555 for c in codes):
-        raise RuntimeError(f"served jobs ended with {codes}")
-    ttr = np.array([r["done"] - r["arrival"] for r in rec])
-    makespan = max(r["done"] for r in rec)
-    cell_steps = sum(r["cells"] * r["steps"] for r in rec)
-    # the same arrivals, one job at a time
-    end, ttr1 = 0.0, []
-    for t, p, _ in sorted(jobs, key=lambda j: j[0]):
-        sh = (p["nthm"], p["nzm"], p["nrm"])
-        ms, setup = alone[sh]
-        start = max(t, end)
-        end = start + setup + (p["ntst"] + 1) * ms / 1e3
-        ttr1.append(end - t)
-    ttr1 = np.array(ttr1)
-    return {
-        "workload": f"{njobs} jobs, Poisson arrivals 2/s, grids 64^2x65..384^2x289, "
-                    "eta/Re/rotation ratio/oscillation seeded, 1000 RK3 steps each",
-        "jobs": njobs, "p50_s": round(float(np.percentile(ttr, 50)), 2),
-        "p99_s": round(float(np.percentile(ttr, 99)), 2),
-        "jobs_per_h": round(njobs / makespan * 3600, 1),
-        "mpt_steps_per_s": round(cell_steps / makespan / 1e6, 1),
-        "one_at_a_time": {"p50_s": round(float(np.percentile(ttr1, 50)), 2),
-                          "p99_s": round(float(np.percentile(ttr1, 99)), 2),
-                          "jobs_per_h": round(njobs / end * 3600, 1)},
-        "speedup_jobs_per_h": round((njobs / makespan) / (njobs / end), 2),
-    }
+    served, m_served = summary(srv.run(jobs))
+    print(f"[bench] served {served}", file=sys.stderr, flush=True)
+    alone, m_alone = summary(sw.run_one_at_a_time(tc, jobs, tuning))
+    print(f"[bench] one at a time {alone}", file=sys.stderr, flush=True)
+    out = {"workload": f"{njobs} jobs, Poisson arrivals 2/s, grids 64^2x65..384^2x289, "
+                       "eta/Re/rotation ratio/oscillation seeded, 1000 RK3 steps each",
+           "jobs": njobs}
+    out.update(served)
+    out["one_at_a_time"] = alone
+    out["speedup_jobs_per_h"] = round(m_alone / m_served, 2)
+    return out
 
 
 def _kernel_short(name):
@@ -580,7 +587,7 @@ def main():
                      "triad_kernel_gbs": round(triad_gbs, 1)},
     }
     if a.sweep_steps > 0:
-        out["sweep"] = sweep_leg(tc, a.sweep_steps)
+        out["sweep"] = sweep_leg(tc, a.sweep_steps, value)
         print(f"[bench] sweep {out['sweep']}", file=sys.stderr, flush=True)
     if a.served_jobs > 0:
         out["served"] = served_leg(tc, a.served_jobs)

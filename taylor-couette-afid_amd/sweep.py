@@ -232,6 +232,44 @@ class _ClassWorker:
         self.b.close()
 
 
+def run_one_at_a_time(tc, jobs, tuning=None, ahead=2):
+    """The baseline of the served sweep: the same arrival trace served one
+    job at a time in arrival order, each job alone on the plain
+    single-instance path (tc_run_steps from its start to NTST).  Host fields
+    are prepared on a worker thread ahead of time, as the server does.
+    Returns one record per job: arrival, admitted, done (s since start),
+    steps, code."""
+    tuning = tuning or {}
+    order = sorted(range(len(jobs)), key=lambda q: jobs[q][0])
+    rec = [None] * len(jobs)
+
+    def prepare(q):
+        p = dict(jobs[q][1])
+        return p, start_fields(tc, p, jobs[q][2])
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, ahead)) as pool:
+        futs = {q: pool.submit(prepare, q) for q in order[:ahead]}
+        t0 = time.perf_counter()
+        for n, q in enumerate(order):
+            for q2 in order[n + 1:n + 1 + ahead]:
+                if q2 not in futs:
+                    futs[q2] = pool.submit(prepare, q2)
+            wait = jobs[q][0] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            admitted = time.perf_counter() - t0
+            p, fields = futs.pop(q).result()
+            sh = (p["nthm"], p["nzm"], p["nrm"])
+            s = tc.Solver(p, tuning=tuning.get(sh))
+            load_and_project(s, fields)
+            del fields
+            code, last, _ = s.run(0, p["ntst"] + 1)
+            s.close()
+            rec[q] = dict(arrival=jobs[q][0], admitted=admitted, done=time.perf_counter() - t0,
+                          steps=last + 1, code=code, cells=p["nthm"] * p["nzm"] * p["nrm"])
+    return rec
+
+
 class Server:
     """Continuous-batching request loop: one resident Batch per grid shape,
     each advanced by its own thread; admission is global."""

@@ -1,0 +1,146 @@
+"""The served sweep's request loop on the CPU (no GPU): admission in arrival
+order by free slot, HBM footprint and resident grid points, backfill past a
+job that does not fit, and every job ending after its own number of steps.
+
+The device batch is replaced by a fake with the same interface that counts
+steps on the host; the host-side set-up (seeded fields, grid vectors, the
+fixed dt) is the library's own.
+"""
+import importlib
+import threading
+import time
+
+import numpy as np
+import pytest
+
+
+class FakeBatch:
+    """tc.Batch stand-in: each advance moves every instance `nsteps` steps
+    (a little wall time per step), ending it at NTST with code 555."""
+
+    log = []
+    lock = threading.Lock()
+
+    def __init__(self):
+        self.inst = {}
+        self.next = 0
+
+    def add(self, p, tuning=None):
+        i = self.next
+        self.next += 1
+        self.inst[i] = dict(p=p, steps=0, code=0)
+        return i
+
+    def solver(self, i):
+        return FakeSolver()
+
+    def advance(self, nsteps, graph=True, sync_every=16):
+        time.sleep(0.001 * nsteps)
+        for i, s in self.inst.items():
+            if s["code"]:
+                continue
+            s["steps"] = min(s["steps"] + nsteps, s["p"]["ntst"] + 1)
+            if s["steps"] == s["p"]["ntst"] + 1:
+                s["code"] = 555
+        with FakeBatch.lock:
+            FakeBatch.log.append(len([s for s in self.inst.values() if not s["code"]]))
+        return sum(1 for s in self.inst.values() if not s["code"])
+
+    def status(self, i):
+        s = self.inst[i]
+        return s["steps"], s["code"], {}
+
+    def remove(self, i):
+        del self.inst[i]
+
+    def close(self):
+        pass
+
+
+class FakeSolver:
+    def set(self, name, a):
+        assert a.ndim == 3
+
+    def project(self):
+        pass
+
+    def get(self, name):
+        return np.zeros((1, 1, 1))
+
+
+class FakeTC:
+    def __init__(self, tc):
+        self._tc = tc
+        self.Batch = FakeBatch
+
+    def __getattr__(self, name):  # host-only library calls pass through
+        return getattr(self._tc, name)
+
+
+def _jobs(sw, grids, arrivals, steps):
+    rng = np.random.default_rng(3)
+    return [(t, sw._instance(rng, g, s), 100 + q)
+            for q, (g, t, s) in enumerate(zip(grids, arrivals, steps))]
+
+
+def test_admission_order_budget_and_backfill(tc):
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    big, small = (48, 48, 33), (16, 16, 9)
+    jobs = _jobs(sw, [big, big, small, small], [0.0, 0.01, 0.02, 0.03], [40, 40, 8, 8])
+    fb, fs = sw.footprint_bytes(jobs[0][1]), sw.footprint_bytes(jobs[2][1])
+    # room for one big job and one small one at a time
+    srv = sw.Server(FakeTC(tc), hbm_budget=fb + fs + 1, chunk=4, max_slots=8, workers=2)
+    rec = srv.run(jobs)
+    for (t, p, _), r in zip(jobs, rec):
+        assert r["code"] == 555 and r["steps"] == p["ntst"] + 1, r
+        assert t <= r["admitted"] <= r["done"], r
+    # the second big job waits for the first to leave; the small jobs are
+    # admitted past it (backfill), the second small one after the first left
+    assert rec[1]["admitted"] >= rec[0]["done"]
+    assert rec[2]["admitted"] < rec[1]["admitted"]
+    assert rec[3]["admitted"] >= rec[2]["done"]
+
+
+def test_slots_and_resident_points_limit_admission(tc):
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    g = (16, 16, 9)
+    jobs = _jobs(sw, [g] * 6, [0.0] * 6, [12] * 6)
+    # two slots: jobs are admitted in arrival order, two at a time
+    rec = sw.Server(FakeTC(tc), chunk=4, max_slots=2, workers=2).run(jobs)
+    assert all(r["code"] == 555 for r in rec)
+    adm = [r["admitted"] for r in rec]
+    assert adm == sorted(adm)
+    for q in range(2, 6):
+        done_before = sorted(r["done"] for r in rec[:q])
+        assert adm[q] >= done_before[q - 2] - 1e-9
+    # a resident-point cap of one job's points: one job at a time
+    cells = g[0] * g[1] * g[2]
+    rec = sw.Server(FakeTC(tc), chunk=4, max_cells=cells, workers=2).run(jobs[:3])
+    for q in (1, 2):
+        assert rec[q]["admitted"] >= rec[q - 1]["done"]
+
+
+def test_one_at_a_time_baseline_serves_in_arrival_order(tc):
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+
+    class Solver(FakeSolver):
+        def __init__(self, p, tuning=None):
+            self.p = p
+
+        def run(self, ntime0, nsteps):
+            time.sleep(0.0005 * nsteps)
+            return 555, self.p["ntst"], {}
+
+        def close(self):
+            pass
+
+    ftc = FakeTC(tc)
+    ftc.Solver = Solver
+    g = (16, 16, 9)
+    jobs = _jobs(sw, [g] * 4, [0.0, 0.05, 0.06, 0.3], [20] * 4)
+    rec = sw.run_one_at_a_time(ftc, jobs)
+    for q, ((t, p, _), r) in enumerate(zip(jobs, rec)):
+        assert r["code"] == 555 and r["steps"] == p["ntst"] + 1
+        assert r["admitted"] >= t
+        if q:
+            assert r["admitted"] >= rec[q - 1]["done"]

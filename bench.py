@@ -556,9 +556,16 @@ def main():
         print(f"[bench] {n:18s} {ms / a.steps:8.3f} ms/step  {ms / cnt:8.4f} ms/launch  "
               f"{gbs:8.1f} GB/s  {100 * ms / max(total_kernel_ms, 1e-9):5.1f}%", file=sys.stderr)
     n_red = 2 if p.get("tout", 1e30) < 1e-6 else 1
-    step_gb = step_bytes_reference(p, n_red) / 1e9
-    print(f"[bench] {ms_step:.3f} ms/step; reference-granularity bytes {step_gb:.1f} GB/step -> "
-          f"{step_gb / (ms_step / 1e3):.0f} GB/s equivalent; last {last}", file=sys.stderr)
+    step_gb_ref = step_bytes_reference(p, n_red) / 1e9
+    # the step's own algorithmic bytes: every kernel launch of the timed
+    # steps with the bytes its launch moves (the fused kernels' intermediates
+    # never reach HBM)
+    step_gb = sum(kb[n] * cnt / a.steps for n, (_, cnt) in ktimes.items() if n in kb) / 1e9
+    step_gbs = step_gb / (ms_step / 1e3)
+    print(f"[bench] {ms_step:.3f} ms/step; fused-path bytes {step_gb:.1f} GB/step = "
+          f"{step_gb / (8 * npts / 1e9):.0f} field passes -> {step_gbs:.0f} GB/s "
+          f"({step_gbs / HBM_PEAK_GBS:.3f} of {HBM_PEAK_GBS:.0f}); the reference's routine "
+          f"granularity would move {step_gb_ref:.1f} GB/step; last {last}", file=sys.stderr)
 
     out = {
         "metric": METRIC,
@@ -585,6 +592,10 @@ def main():
                      "bytes_per_launch": kb[dom],
                      "copy_kernel_gbs": round(copy_gbs, 1),
                      "triad_kernel_gbs": round(triad_gbs, 1)},
+        "step_roofline": {"bound": "hbm", "bytes_per_step": round(step_gb * 1e9),
+                          "field_passes": round(step_gb / (8 * npts / 1e9), 1),
+                          "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }
     if a.sweep_steps > 0:
         out["sweep"] = sweep_leg(tc, a.sweep_steps, value)

@@ -461,7 +461,9 @@ int tc_batch_advance(tc_batch *b, int nsteps, int use_graph, int sync_every) {
     BCHK(hipEventRecord(b->dyn_ev[k], b->stream));
     if (use_graph) {
       if (!b->exec) {
-        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        // relaxed: other threads (the served loop's other shape classes)
+        // keep creating contexts and uploading fields while one captures
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
         rc = enqueue_step(b);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(b->stream, &graph);

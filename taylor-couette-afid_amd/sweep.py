@@ -24,8 +24,9 @@ fits the HBM budget and the resident grid points stay under ``max_cells``
 only slow the others down).  A job that does not fit may be overtaken by a
 later one that does (backfill).  Admitted jobs stay resident in one
 ``Batch`` per grid shape, each advanced by its own thread on its own
-streams, so small jobs do not wait for the step of a large one; jobs join
-and leave a batch at step boundaries, every ``chunk`` steps.  Host fields
+streams, so small jobs do not wait for the step of a large one; each
+batch's step (all its grouped launches) is replayed as one hipGraph; jobs
+join and leave a batch at step boundaries, every ``chunk`` steps.  Host fields
 are prepared on worker threads while the device computes, and each job's
 time to result (arrival to final step) is recorded.
 """
@@ -270,21 +271,31 @@ def run_one_at_a_time(tc, jobs, tuning=None, ahead=2):
     return rec
 
 
+def _work(p):
+    """Grid points times steps of one job."""
+    return p["nthm"] * p["nzm"] * p["nrm"] * (p["ntst"] + 1)
+
+
 class Server:
     """Continuous-batching request loop: one resident Batch per grid shape,
     each advanced by its own thread; admission is global."""
 
-    def __init__(self, tc, hbm_budget=200e9, chunk=25, graph=False, tuning=None, workers=4,
-                 max_slots=16, max_cells=400e6):
-        # graph=False: the class workers run on their own threads, and HIP
-        # refuses legacy-stream copies (field uploads) while another thread
-        # captures a graph; eager grouped launches measure the same here
+    def __init__(self, tc, hbm_budget=200e9, chunk=25, graph=True, tuning=None, workers=4,
+                 max_slots=16, max_cells=400e6, order="arrival"):
+        # graph=True: each class worker replays its batch's whole step as a
+        # hipGraph, captured again whenever jobs join or leave
         self.tc = tc
         self.budget = hbm_budget
         self.max_slots = max_slots
         self.max_cells = max_cells
         self.chunk = chunk
         self.graph = graph
+        # "arrival": arrived jobs are considered in arrival order;
+        # "shortest": the least work (grid points x steps) first, arrival
+        # order among equals
+        if order not in ("arrival", "shortest"):
+            raise ValueError(f"unknown admission order {order!r}")
+        self.order = order
         self.tuning = tuning or {}
         self.pool = cf.ThreadPoolExecutor(max_workers=workers)
         self.ahead = workers + 2
@@ -329,11 +340,14 @@ class Server:
                     if q not in futures:
                         futures[q] = self.pool.submit(self._prepare, jobs[q][1], jobs[q][2])
                 now = time.perf_counter() - self.t0
-                # admission: arrived jobs in arrival order, each if a slot,
-                # the HBM budget and the resident-cell target allow (always
-                # at least one resident job); later jobs may backfill past
-                # one that waits
-                for q in [q for q in waiting if jobs[q][0] <= now]:
+                # admission: arrived jobs in the admission order, each if a
+                # slot, the HBM budget and the resident-cell target allow
+                # (always at least one resident job); later jobs may backfill
+                # past one that waits
+                arrived = [q for q in waiting if jobs[q][0] <= now]
+                if self.order == "shortest":
+                    arrived.sort(key=lambda q: (_work(jobs[q][1]), jobs[q][0]))
+                for q in arrived:
                     p0 = jobs[q][1]
                     need = footprint_bytes(p0)
                     c = p0["nthm"] * p0["nzm"] * p0["nrm"]

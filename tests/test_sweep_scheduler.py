@@ -144,3 +144,15 @@ def test_one_at_a_time_baseline_serves_in_arrival_order(tc):
         assert r["admitted"] >= t
         if q:
             assert r["admitted"] >= rec[q - 1]["done"]
+
+
+def test_shortest_first_order_admits_small_jobs_first(tc):
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    big, small = (48, 48, 33), (16, 16, 9)
+    jobs = _jobs(sw, [big, big, small, small], [0.0, 0.0, 0.0, 0.0], [40, 40, 8, 8])
+    rec = sw.Server(FakeTC(tc), chunk=4, max_slots=1, order="shortest", workers=2).run(jobs)
+    adm = [r["admitted"] for r in rec]
+    assert max(adm[2], adm[3]) < min(adm[0], adm[1])
+    assert adm[0] < adm[1] and adm[2] < adm[3]  # arrival order among equals
+    with pytest.raises(ValueError):
+        sw.Server(FakeTC(tc), order="random")

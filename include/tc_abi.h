@@ -159,6 +159,17 @@ int tc_set_fused(tc_ctx *ctx, int fused);
 
 int tc_calc_max_cfl(tc_ctx *ctx, double *cflm);        /* CalcMaxCFL.F90:12-49     */
 int tc_check_divergence(tc_ctx *ctx, double *qmax);    /* CheckDivergence.F90:11-46 */
+
+/* tc_check_divergence without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The reduction's per-block partials live in
+ * a fixed 512 KiB device array of the library (one per GPU, kept until the
+ * process ends) instead of the context's buffer, and the result, max|div|,
+ * is written to qmax_dev, a double in device memory, asynchronously on the
+ * context's stream (read it after tc_synchronize).  Planes of more than
+ * 16,777,216 points (nzm*nthm) are rejected.  Calls that overlap on one GPU
+ * share that array and race. */
+int tc_check_divergence_noalloc(tc_ctx *ctx, double *qmax_dev);
 int tc_calc_torque(tc_ctx *ctx, double *inner, double *outer); /* CalcTorque.F90:12-62 */
 int tc_calc_velocity_stats(tc_ctx *ctx, double *vmax3, double *rms3); /* CalcVelocityStats.F90:11-68 */
 

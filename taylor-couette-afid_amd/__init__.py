@@ -64,7 +64,7 @@ ABI_SYMBOLS = [
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
     "tc_regrid_field", "tc_stream_triad", "tc_kernel_timing_only", "tc_run_steps",
-    "tc_run_steps_logged",
+    "tc_run_steps_logged", "tc_check_divergence_noalloc",
 ]
 
 # stat_arrays in WriteStats order (StatRoutines.F90:97-152); TC_ST_* ids
@@ -140,6 +140,7 @@ def lib():
         getattr(L, n).argtypes = [_V]
     L.tc_calc_max_cfl.argtypes = [_V, _D]
     L.tc_check_divergence.argtypes = [_V, _D]
+    L.tc_check_divergence_noalloc.argtypes = [_V, ctypes.c_void_p]
     L.tc_calc_torque.argtypes = [_V, _D, _D]
     L.tc_calc_velocity_stats.argtypes = [_V, _D, _D]
     L.tc_step.argtypes = [_V, ctypes.c_int, ctypes.c_double, _D]

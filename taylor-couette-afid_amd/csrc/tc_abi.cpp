@@ -994,6 +994,12 @@ int tc_check_divergence(tc_ctx *c, double *qmax) {
   return read_reduction(c, 1, qmax);
 }
 
+int tc_check_divergence_noalloc(tc_ctx *c, double *qmax_dev) {
+  CTXCHK(c);
+  if (!qmax_dev) return fail("tc_check_divergence_noalloc: null output");
+  return tc::check_divergence_noalloc(c, qmax_dev);
+}
+
 int tc_calc_torque(tc_ctx *c, double *inner, double *outer) {
   CTXCHK(c);
   tc::launch_torque(c);

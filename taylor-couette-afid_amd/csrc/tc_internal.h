@@ -233,6 +233,7 @@ void launch_velocity_stats(tc_ctx *c);
 // one pass: cfl, max|div|, vmax(3), rms sums(3), torque sums(2)
 void launch_diagnostics(tc_ctx *c);
 bool correct_diag_applies(const tc_ctx *c);
+int check_divergence_noalloc(tc_ctx *c, double *qmax_dev);
 void launch_correct_diag(tc_ctx *c);
 void launch_stats(tc_ctx *c, bool stats, bool diss);
 void launch_dump(tc_ctx *c, int kind, int k);

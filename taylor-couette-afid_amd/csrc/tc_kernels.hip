@@ -5,6 +5,9 @@
 // r-plane k (1-based, k = 1..nr) at offset (k-1)*P, P = nzm*nthm.  The
 // reference's periodic halos (InitVariables.F90:26-35, halo_common.f90:67-108)
 // become index wrap-around, so no halo update is ever launched.
+#include <map>
+#include <mutex>
+
 #include "tc_internal.h"
 
 namespace tc {
@@ -1482,6 +1485,60 @@ void launch_max_div(tc_ctx *c) {
   finish_reduction(c, 1, 1u);
   timer_end(c);
 }
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point tc_check_divergence_noalloc: the
+// per-block partials of its divergence reduction live in this fixed device
+// array instead of the context's reduction buffer.  Each GPU the library
+// runs on holds its own copy, at its own address, until the process ends;
+// in host code `scratch` names no device memory, so host code uses only the
+// address looked up per device below.  Every block writes its own partial
+// before the combine reads it, so nothing in it needs clearing; calls that
+// overlap on one GPU race on it.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 65536;  // 512 KiB: planes up to 16 Mi points
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+static std::mutex scratch_mu;
+static std::map<int, double *> scratch_dev;  // device -> address of scratch there
+
+// The device address of scratch on device dev, looked up once per device
+// with that device current.
+static int scratch_on(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  const auto it = scratch_dev.find(dev);
+  if (it != scratch_dev.end()) {
+    *out = it->second;
+    return 0;
+  }
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return set_error("hipGetDevice failed");
+  if (cur != dev && hipSetDevice(dev) != hipSuccess) return set_error("hipSetDevice failed");
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !ptr)
+    return set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  scratch_dev[dev] = static_cast<double *>(ptr);
+  *out = static_cast<double *>(ptr);
+  return 0;
+}
+
+int check_divergence_noalloc(tc_ctx *c, double *qmax_dev) {
+  const unsigned nb = red_blocks(c);
+  if ((size_t)nb > NOALLOC_SCRATCH_DOUBLES)
+    return set_error("tc_check_divergence_noalloc: a plane of " + std::to_string(c->P) +
+                     " points needs " + std::to_string(nb) + " partials; the fixed scratch holds " +
+                     std::to_string(NOALLOC_SCRATCH_DOUBLES) + " (nzm*nthm <= " +
+                     std::to_string(NOALLOC_SCRATCH_DOUBLES * RED_THREADS) + ")");
+  double *part = nullptr;
+  int rc = scratch_on(c->device, &part);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pointmax<1>, dim3(nb), dim3(RED_THREADS), 0, c->stream, c->gd, c->f[TC_QT],
+                     c->f[TC_QR], c->f[TC_QZ], part);
+  hipLaunchKernelGGL(k_combine, dim3(1u), dim3(256), 0, c->stream, part, (int)nb, 1, 1u, qmax_dev);
+  return check_launch();
+}
+
 void launch_torque(tc_ctx *c) {
   const double del = -1.0 / (c->g.rc[2] - c->g.rc[1]);
   const double deln = -1.0 / (c->g.rc[c->nr] - c->g.rc[c->nrm]);

@@ -623,34 +623,73 @@ DEVI void poisson_body(const long long blk, const GridDev &g, double2 *__restric
   rl[(2 * pp + 1) * 4 + 2] = dL.x;
   rl[(2 * pp + 1) * 4 + 3] = dL.y;
   __syncthreads();
-  if (pp == 0) {
-    // unit-diagonal tridiagonal system of the 2*PT boundary unknowns
-    // (row r+1 is read before row r is written: the reads do not wait on
-    // the stores)
-    const int R = 2 * PT;
+  // Unit-diagonal tridiagonal system of the 2*PT boundary unknowns, solved
+  // from both ends at once: thread pp = 0 eliminates rows 0..PT-1 downward
+  // (x_r + c'_r x_(r+1) = d'_r), pp = 1 rows 2PT-1..PT upward (the same
+  // recurrence on the reversed rows: a''_r x_(r-1) + x_r = d''_r); the two
+  // meet in a 2x2 system for x_(PT-1), x_PT, and each substitutes back
+  // through its own half.  Both run one instruction stream, so they share a
+  // wave's issue slots, and the serial chain is half as long.  (Row r+1 is
+  // read before row r is written: the reads do not wait on the stores.)
+  const int R = 2 * PT, mrow = PT;
+  const bool up = (pp == 1);
+  if (PT == 1) {
+    // one partition (nrm < 4): its two boundary rows by one thread
+    if (pp == 0) {
+      const double4 t = *(const double4 *)(rl), b = *(const double4 *)(rl + 4);
+      const double piv = 1.0 / (1.0 - b.x * t.y);
+      const double x1 = (b.z - b.x * t.z) * piv, y1 = (b.w - b.x * t.w) * piv;
+      rl[2] = t.z - t.y * x1;
+      rl[3] = t.w - t.y * y1;
+      rl[6] = x1;
+      rl[7] = y1;
+    }
+  } else {
+  if (pp < 2) {
+    const int step = up ? -1 : 1;
+    int r = up ? R - 1 : 0;
     double cprev = 0.0, dxp = 0.0, dyp = 0.0;
-    double4 nx = *(const double4 *)(rl);
-    for (int r = 0; r < R; r++) {
+    double4 nx = *(const double4 *)(rl + r * 4);
+    for (int q = 0; q < mrow; q++, r += step) {
       const double4 cur = nx;
-      if (r + 1 < R) nx = *(const double4 *)(rl + (r + 1) * 4);
-      const double q = 1.0 / (1.0 - cur.x * cprev);
-      cprev = cur.y * q;
-      dxp = (cur.z - cur.x * dxp) * q;
-      dyp = (cur.w - cur.x * dyp) * q;
-      rl[r * 4 + 1] = cprev;
+      if (q + 1 < mrow) nx = *(const double4 *)(rl + (r + step) * 4);
+      const double sub = up ? cur.y : cur.x, sup = up ? cur.x : cur.y;
+      const double piv = 1.0 / (1.0 - sub * cprev);
+      cprev = sup * piv;
+      dxp = (cur.z - sub * dxp) * piv;
+      dyp = (cur.w - sub * dyp) * piv;
+      rl[r * 4 + (up ? 0 : 1)] = cprev;
       rl[r * 4 + 2] = dxp;
       rl[r * 4 + 3] = dyp;
     }
-    double xx = dxp, xy = dyp;
-    double4 pv = (R >= 2) ? *(const double4 *)(rl + (R - 2) * 4) : make_double4(0, 0, 0, 0);
-    for (int r = R - 2; r >= 0; r--) {
-      const double4 cur = pv;
-      if (r >= 1) pv = *(const double4 *)(rl + (r - 1) * 4);
-      xx = cur.z - cur.y * xx;
-      xy = cur.w - cur.y * xy;
+  }
+  __syncthreads();
+  double xx = 0.0, xy = 0.0;
+  if (pp < 2) {
+    // rows PT-1 (x + c' x_PT = d') and PT (a'' x_(PT-1) + x = d'')
+    const double4 t = *(const double4 *)(rl + (mrow - 1) * 4);
+    const double4 b = *(const double4 *)(rl + mrow * 4);
+    const double piv = 1.0 / (1.0 - t.y * b.x);
+    const double x0 = (t.z - t.y * b.z) * piv, y0 = (t.w - t.y * b.w) * piv;  // x_(PT-1)
+    xx = up ? b.z - b.x * x0 : x0;                                              // x_PT
+    xy = up ? b.w - b.x * y0 : y0;
+  }
+  __syncthreads();  // both meeting rows are read before either is overwritten
+  if (pp < 2) {
+    const int step = up ? 1 : -1;
+    int r = up ? mrow : mrow - 1;
+    rl[r * 4 + 2] = xx;
+    rl[r * 4 + 3] = xy;
+    for (int q = 1; q < mrow; q++) {
+      r += step;
+      const double4 cur = *(const double4 *)(rl + r * 4);
+      const double cf = up ? cur.x : cur.y;  // a''_r or c'_r
+      xx = cur.z - cf * xx;
+      xy = cur.w - cf * xy;
       rl[r * 4 + 2] = xx;
       rl[r * 4 + 3] = xy;
     }
+  }
   }
   if (gblock && tid == nthr - 1) {
     // gauge line: Thomas on rows 1..nrm-1 (b = acph there), phi(nrm) = 0

@@ -1649,9 +1649,9 @@ void launch_diagnostics(tc_ctx *c) {
 // that every raw read (neighbours, and qr(k+1) across r segments) sees the
 // raw value, the corrected velocity goes to the three spare arrays tout,
 // rout, zout (the caller then swaps the field pointers); pr is corrected in
-// place (no pressure neighbour is read).  Blocks: 64 pairs x 4 rows,
-// gridDim.z = segments of r; per-block partials of the ten diagnostics in
-// part (diagnostics_body order).
+// place (no pressure neighbour is read).  Blocks: 64 pairs x 4 rows, one
+// segment of r each; per-block partials of the ten diagnostics in part
+// (diagnostics_body order).
 // ---------------------------------------------------------------------------
 DEVI void correct_diag_body(const int bx, const int by, const int seg, const int nseg,
                             const GridDev &g, const double *__restrict__ qt,
@@ -1662,7 +1662,7 @@ DEVI void correct_diag_body(const int bx, const int by, const int seg, const int
                             const unsigned char *__restrict__ icyl,
                             const unsigned char *__restrict__ ocyl, double virotint,
                             double virotext, double del, double deln, double pi,
-                            double *__restrict__ part, int blk) {
+                            double *__restrict__ part, int blk, int sync) {
 #pragma clang fp contract(off)
   double v[10] = {0.0, 0.0, -100.0, -100.0, -100.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int lane = threadIdx.x;
@@ -1784,6 +1784,10 @@ DEVI void correct_diag_body(const int bx, const int by, const int seg, const int
     dkm = d;
     d = dkp;
     rraw = rnext;
+    // every sync planes the block's four rows meet, so that each row's dph
+    // and qz neighbours are read while the wave that owns them has them in
+    // cache (many planes apart they are gone from L2)
+    if (sync > 0 && (kc - k0) % sync == sync - 1) __syncthreads();
   }
   const int ops[10] = {1, 1, 1, 1, 1, 0, 0, 0, 0, 0};
   block_reduce<10>(v, ops, part + 10 * blk);
@@ -1795,26 +1799,48 @@ __global__ __launch_bounds__(256) void k_correct_diag(
     double *__restrict__ rout, double *__restrict__ zout, const double *__restrict__ dph,
     double dt, double al, double be, const unsigned char *__restrict__ icyl,
     const unsigned char *__restrict__ ocyl, double virotint, double virotext, double del,
-    double deln, double pi, double *__restrict__ part) {
-  const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  correct_diag_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, g, qt, qr, qz, pr, tout, rout,
-                    zout, dph, dt, al, be, icyl, ocyl, virotint, virotext, del, deln, pi, part,
-                    blk);
+    double deln, double pi, double *__restrict__ part, int sync, int xcdmap) {
+  // one-dimensional grid of (theta block, z block, r segment) tiles, theta
+  // fastest, handed out XCD by XCD so that z-neighbouring tiles share an L2
+  const int nbx = (g.nthm / 2 + 63) / 64, nby = (g.nzm + 3) / 4;
+  const int blk = xcdmap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int bx = blk % nbx, by = (blk / nbx) % nby, seg = blk / (nbx * nby);
+  correct_diag_body(bx, by, seg, gridDim.x / (nbx * nby), g, qt, qr, qz, pr, tout, rout, zout, dph,
+                    dt, al, be, icyl, ocyl, virotint, virotext, del, deln, pi, part, blk, sync);
 }
 
-// Segments of r per column: enough blocks to fill the device about four
-// blocks per CU, each segment at least 16 planes, and the partials within
-// the reduction buffer (TC_CDIAG_SEG overrides).
+// Segments of r per column: enough blocks for about six per CU (at
+// 512x512x385 three segments of 129 planes ran 1.46-1.51 ms against 1.81 ms
+// for two and 1.56-1.61 ms for four to twelve), each segment at least 16
+// planes, and the partials within the reduction buffer (TC_CDIAG_SEG
+// overrides).
 int correct_diag_segments(const tc_ctx *c) {
   const long long cols = (long long)((c->nthm / 2 + 63) / 64) * ((c->nzm + 3) / 4);
   static const int env = [] {
     const char *e = std::getenv("TC_CDIAG_SEG");
     return e ? std::atoi(e) : 0;
   }();
-  long long s = env > 0 ? env : (1024 + cols - 1) / cols;
+  long long s = env > 0 ? env : (1536 + cols - 1) / cols;
   s = std::min<long long>(s, std::max(1, c->nrm / 16));
   s = std::min<long long>(s, (long long)c->red_cap / (10 * cols));
   return (int)std::max<long long>(s, 1);
+}
+
+// Planes between the meetings of a block's rows (TC_CDIAG_SYNC; 0: never),
+// and whether tiles go out XCD by XCD (TC_CDIAG_XCD; default 1).
+static int correct_diag_sync() {
+  static const int v = [] {
+    const char *e = std::getenv("TC_CDIAG_SYNC");
+    return e ? std::atoi(e) : 0;
+  }();
+  return v;
+}
+static int correct_diag_xcd() {
+  static const int v = [] {
+    const char *e = std::getenv("TC_CDIAG_XCD");
+    return e ? std::atoi(e) : 1;
+  }();
+  return v;
 }
 
 bool correct_diag_applies(const tc_ctx *c) { return c->nthm % 2 == 0 && c->nrm >= 3; }
@@ -1823,19 +1849,20 @@ bool correct_diag_applies(const tc_ctx *c) { return c->nthm % 2 == 0 && c->nrm >
 // the end of a substep), whose pointers then trade places with qt, qr, qz.
 void launch_correct_diag(tc_ctx *c) {
   const int nseg = correct_diag_segments(c);
-  const dim3 grid((unsigned)((c->nthm / 2 + 63) / 64), (unsigned)((c->nzm + 3) / 4), (unsigned)nseg);
+  const dim3 grid((unsigned)(((c->nthm / 2 + 63) / 64) * ((c->nzm + 3) / 4) * nseg));
   const double del = -1.0 / (c->g.rc[2] - c->g.rc[1]);
   const double deln = -1.0 / (c->g.rc[c->nr] - c->g.rc[c->nrm]);
   timer_begin(c, "correct_diag");
   hipLaunchKernelGGL(k_correct_diag, grid, dim3(64, 4), 0, c->stream, c->gd, c->f[TC_QT],
                      c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR], c->f[TC_RHS], c->f[TC_DQ],
                      c->f[TC_QCAP], c->f[TC_DPH], c->dt, c->al, c->beta, c->mask, c->mask + c->P,
-                     c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red);
+                     c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red, correct_diag_sync(),
+                     correct_diag_xcd());
   std::swap(c->f[TC_QT], c->f[TC_RHS]);
   std::swap(c->f[TC_QR], c->f[TC_DQ]);
   std::swap(c->f[TC_QZ], c->f[TC_QCAP]);
-  hipLaunchKernelGGL(k_combine, dim3(10u), dim3(256), 0, c->stream, c->red,
-                     (int)(grid.x * grid.y * grid.z), 10, 0x1Fu, c->red_out);
+  hipLaunchKernelGGL(k_combine, dim3(10u), dim3(256), 0, c->stream, c->red, (int)grid.x, 10,
+                     0x1Fu, c->red_out);
   hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * 10, hipMemcpyDeviceToHost, c->stream);
   timer_end(c);
 }

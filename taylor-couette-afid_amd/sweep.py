@@ -315,6 +315,7 @@ class Server:
                 self.used -= footprint_bytes(self.jobs[q][1])
                 self.cells -= r["cells"]
                 self.nactive -= 1
+            self.events += 1
             self.cv.notify_all()
 
     def run(self, jobs, keep_fields=False):
@@ -324,6 +325,7 @@ class Server:
         self.jobs, self.keep_fields = jobs, keep_fields
         self.rec = [None] * len(jobs)
         self.used, self.cells, self.nactive = 0.0, 0, 0
+        self.events = 0  # jobs finished (or workers failed) so far
         self.t0 = time.perf_counter()
         workers = {}
         waiting = sorted(range(len(jobs)), key=lambda q: jobs[q][0])
@@ -336,6 +338,7 @@ class Server:
                             raise w.error
                     if not waiting and self.nactive == 0:
                         break
+                    seen = self.events
                 for q in waiting[: self.ahead]:
                     if q not in futures:
                         futures[q] = self.pool.submit(self._prepare, jobs[q][1], jobs[q][2])
@@ -347,11 +350,15 @@ class Server:
                 arrived = [q for q in waiting if jobs[q][0] <= now]
                 if self.order == "shortest":
                     arrived.sort(key=lambda q: (_work(jobs[q][1]), jobs[q][0]))
-                for q in arrived:
-                    p0 = jobs[q][1]
-                    need = footprint_bytes(p0)
-                    c = p0["nthm"] * p0["nzm"] * p0["nrm"]
-                    with self.cv:
+                # one decision over one snapshot of the resident set: a job
+                # finishing meanwhile must not let a later job of the same
+                # size pass one that was just turned away
+                take = []
+                with self.cv:
+                    for q in arrived:
+                        p0 = jobs[q][1]
+                        need = footprint_bytes(p0)
+                        c = p0["nthm"] * p0["nzm"] * p0["nrm"]
                         if self.nactive >= self.max_slots:
                             break
                         if self.nactive and (self.used + need > self.budget or
@@ -360,6 +367,9 @@ class Server:
                         self.used += need
                         self.cells += c
                         self.nactive += 1
+                        take.append((q, c))
+                for q, c in take:
+                    p0 = jobs[q][1]
                     if q not in futures:
                         futures[q] = self.pool.submit(self._prepare, p0, jobs[q][2])
                     waiting.remove(q)
@@ -372,11 +382,13 @@ class Server:
                                            steps_total=p["ntst"] + 1)
                     workers[sh].admit(q, p, fields, self.tuning.get(sh))
                     del fields
-                # sleep until a job finishes or the next arrival
+                # sleep until a job finishes or the next arrival; a job that
+                # finished during this pass wakes the loop at once
                 with self.cv:
                     nxt = min((jobs[q][0] for q in waiting), default=None)
                     timeout = 0.5 if nxt is None else max(0.0, nxt - (time.perf_counter() - self.t0))
-                    self.cv.wait(min(timeout, 0.5))
+                    if self.events == seen:
+                        self.cv.wait(min(timeout, 0.5))
         finally:
             for w in workers.values():
                 w.close()

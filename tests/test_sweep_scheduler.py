@@ -156,3 +156,20 @@ def test_shortest_first_order_admits_small_jobs_first(tc):
     assert adm[0] < adm[1] and adm[2] < adm[3]  # arrival order among equals
     with pytest.raises(ValueError):
         sw.Server(FakeTC(tc), order="random")
+
+
+def test_admission_order_holds_when_jobs_finish_during_a_pass(tc):
+    """A job finishing while the server walks the waiting list must not let a
+    later job of the same size pass one that was just turned away: each pass
+    decides on one snapshot of the resident set (repeated, since the finish
+    has to fall between two admission checks)."""
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    g = (16, 16, 9)
+    jobs = _jobs(sw, [g] * 3, [0.0] * 3, [12] * 3)
+    cells = g[0] * g[1] * g[2]
+    for _ in range(20):
+        rec = sw.Server(FakeTC(tc), chunk=4, max_cells=cells, workers=2).run(jobs)
+        adm = [r["admitted"] for r in rec]
+        assert adm == sorted(adm), rec
+        for q in (1, 2):
+            assert rec[q]["admitted"] >= rec[q - 1]["done"], rec

@@ -130,11 +130,38 @@ struct BatchRes {
 // loop does, and takes no further step: every grouped kernel leaves at once
 // for it from the next step on.
 #ifdef __HIPCC__
+// A pointer read from a descriptor table in device memory is generic to the
+// compiler, which then addresses it with flat instructions and 64-bit VGPR
+// addresses (about 50 more VGPRs and one wave less per SIMD in the grouped
+// Helmholtz kernel than in its plain twin, whose pointers are kernel
+// arguments).  Every pointer in a StepDesc is device memory; passing it
+// through an integer into the global address space gives global loads with
+// scalar bases, as kernel arguments get.
+template <class T>
+__device__ __forceinline__ T *glob(T *p) {
+  typedef __attribute__((address_space(1))) T GT;
+  return (T *)(GT *)(unsigned long long)p;
+}
+__device__ __forceinline__ GridDev glob(const GridDev &s) {
+  GridDev g = s;
+  g.hp = glob(g.hp);
+  g.rc = glob(g.rc), g.rm = glob(g.rm), g.usrc = glob(g.usrc), g.usrm = glob(g.usrm);
+  g.udx2rm = glob(g.udx2rm), g.udx2m = glob(g.udx2m), g.udx2c = glob(g.udx2c);
+  g.rudx2c = glob(g.rudx2c), g.udx1vis1 = glob(g.udx1vis1), g.udx1vis2 = glob(g.udx1vis2);
+  g.udx1qm = glob(g.udx1qm), g.pvol = glob(g.pvol);
+  g.ap1j = glob(g.ap1j), g.am1j = glob(g.am1j), g.ac1j = glob(g.ac1j);
+  g.ap2je = glob(g.ap2je), g.am2je = glob(g.am2je), g.ac2je = glob(g.ac2je);
+  g.ap3j = glob(g.ap3j), g.am3j = glob(g.am3j), g.ac3j = glob(g.ac3j);
+  g.apph = glob(g.apph), g.amph = glob(g.amph), g.acph = glob(g.acph);
+  g.ak1 = glob(g.ak1), g.ak3 = glob(g.ak3), g.udx2mm = glob(g.udx2mm), g.udx2cc = glob(g.udx2cc);
+  return g;
+}
 __device__ __forceinline__ bool inst_stopped(const StepDesc &d) {
   if (!d.res) return false;
-  const int a = d.res->fail165, b = d.res->faillog;
+  const BatchRes *r = glob(d.res);
+  const int a = r->fail165, b = r->faillog;
   const int f = (a && b) ? (a < b ? a : b) : (a ? a : b);
-  return f != 0 && d.dyn->ntime >= f;
+  return f != 0 && glob(d.dyn)->ntime >= f;
 }
 #endif
 

@@ -501,9 +501,10 @@ __global__ __launch_bounds__(256) void k_helm_bwd(GridDev g, double *__restrict_
 __global__ __launch_bounds__(256) void k_helm_bwd_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  helm_bwd_body<true>((long long)blockIdx.x * blockDim.x + threadIdx.x, d.g, d.f[TC_QT], d.f[TC_QR],
-                d.f[TC_QZ], d.f[TC_RHS], d.f[TC_DQ], d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur,
-                d.luz, d.lu_nt, 7);
+  helm_bwd_body<true>((long long)blockIdx.x * blockDim.x + threadIdx.x, glob(d.g),
+                      glob(d.f[TC_QT]), glob(d.f[TC_QR]), glob(d.f[TC_QZ]), glob(d.f[TC_RHS]),
+                      glob(d.f[TC_DQ]), glob(d.f[TC_QCAP]), glob(d.icyl), glob(d.ocyl),
+                      glob(d.lut), glob(d.lur), glob(d.luz), d.lu_nt, 7);
 }
 
 // ---------------------------------------------------------------------------
@@ -885,13 +886,14 @@ __global__ __launch_bounds__(256) void k_helm_fwd_g(const StepDesc *__restrict__
   const StepDesc &d = tab[inst];
   if (inst_stopped(d)) return;
   ImpArgs a = d.a;
-  a.vwall_in = dyn[inst].vwall[ns - 1];
-  a.bc1 = dyn[inst].bc1[ns - 1];
+  a.vwall_in = glob(dyn)[inst].vwall[ns - 1];
+  a.bc1 = glob(dyn)[inst].bc1[ns - 1];
   const CorrIn ci{d.dt, d.corr_al, d.beta};
-  helm_fwd_body<UNIFORM, LPT, CORR>(tile, d.g, a, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.pr_in,
-                                    d.f[TC_RU1], d.f[TC_RU2], d.f[TC_RU3], d.f[TC_RHS], d.f[TC_DQ],
-                                    d.f[TC_QCAP], d.icyl, d.ocyl, d.lut, d.lur, d.luz, d.lu_nt, ntx,
-                                    ci, d.f[TC_DPH], d.pr_out);
+  helm_fwd_body<UNIFORM, LPT, CORR>(
+      tile, glob(d.g), a, glob(d.f[TC_QT]), glob(d.f[TC_QR]), glob(d.f[TC_QZ]), glob(d.pr_in),
+      glob(d.f[TC_RU1]), glob(d.f[TC_RU2]), glob(d.f[TC_RU3]), glob(d.f[TC_RHS]), glob(d.f[TC_DQ]),
+      glob(d.f[TC_QCAP]), glob(d.icyl), glob(d.ocyl), glob(d.lut), glob(d.lur), glob(d.luz),
+      d.lu_nt, ntx, ci, glob(d.f[TC_DPH]), glob(d.pr_out));
 }
 
 static ImpArgs imp_args(const tc_ctx *c) {
@@ -1182,8 +1184,8 @@ __global__ __launch_bounds__(256) void k_correct_both2_g(const StepDesc *__restr
   const int i = 2 * (blockIdx.x * 64 + threadIdx.x);
   const int j = blockIdx.y * 4 + threadIdx.y;
   if (i >= d.g.nthm || j >= d.g.nzm) return;
-  correct_pair(d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.f[TC_PR], d.f[TC_DPH], kc, j, i, d.dt, d.al,
-               d.beta);
+  correct_pair(glob(d.g), glob(d.f[TC_QT]), glob(d.f[TC_QR]), glob(d.f[TC_QZ]), glob(d.f[TC_PR]),
+               glob(d.f[TC_DPH]), kc, j, i, d.dt, d.al, d.beta);
 }
 
 // Grouped: blockIdx.z = instance * nrm + plane.
@@ -1191,13 +1193,14 @@ __global__ __launch_bounds__(256) void k_correct_both_g(const StepDesc *__restri
   const int inst = blockIdx.z / nrm, kc = blockIdx.z - inst * nrm + 1;
   const StepDesc &d = tab[inst];
   if (inst_stopped(d)) return;
-  const GridDev &g = d.g;
+  const GridDev g = glob(d.g);
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y * 4 + threadIdx.y;
   if (i >= g.nthm || j >= g.nzm) return;
-  double *qr = d.f[TC_QR];
-  correct_velocity_at(g, d.f[TC_QT], qr, d.f[TC_QZ], d.f[TC_DPH], kc, j, i, d.dt, d.al);
-  correct_pressure_at(g, d.f[TC_PR], d.f[TC_DPH], kc, j, i, d.beta);
+  double *qr = glob(d.f[TC_QR]);
+  correct_velocity_at(g, glob(d.f[TC_QT]), qr, glob(d.f[TC_QZ]), glob(d.f[TC_DPH]), kc, j, i, d.dt,
+                      d.al);
+  correct_pressure_at(g, glob(d.f[TC_PR]), glob(d.f[TC_DPH]), kc, j, i, d.beta);
   if (kc == 1) qr[IX(g, 1, j, i)] = 0.0;
   if (kc == g.nrm) qr[IX(g, g.nr, j, i)] = 0.0;
 }
@@ -1334,7 +1337,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_pointmax(GridDev g, const doubl
 __global__ __launch_bounds__(RED_THREADS) void k_pointmax_cfl_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  pointmax_body<0>(blockIdx.x, d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.red);
+  pointmax_body<0>(blockIdx.x, glob(d.g), glob(d.f[TC_QT]), glob(d.f[TC_QR]), glob(d.f[TC_QZ]),
+                   glob(d.red));
 }
 
 // CalcTorque.F90:36-45 masked wall sums
@@ -1423,7 +1427,7 @@ __global__ __launch_bounds__(256) void k_combine_g(const StepDesc *__restrict__ 
   if (dyn && !dyn[blockIdx.y].log) return;
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  combine_body(blockIdx.x, d.red, nb, nv, opmask, d.red_out);
+  combine_body(blockIdx.x, glob(d.red), nb, nv, opmask, glob(d.red_out));
 }
 
 // CalcMaxCFL floor and the fixed-dt guard of main.F90:156,172-173, one
@@ -1621,8 +1625,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_diagnostics_g(const StepDesc *_
   if (!dyn[blockIdx.y].log) return;
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  diagnostics_body(blockIdx.x, d.g, d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.icyl, d.ocyl,
-                   d.virotint, d.virotext, d.del, d.deln, d.pi, d.red);
+  diagnostics_body(blockIdx.x, glob(d.g), glob(d.f[TC_QT]), glob(d.f[TC_QR]), glob(d.f[TC_QZ]),
+                   glob(d.icyl), glob(d.ocyl), d.virotint, d.virotext, d.del, d.deln, d.pi,
+                   glob(d.red));
 }
 
 void launch_diagnostics(tc_ctx *c) {

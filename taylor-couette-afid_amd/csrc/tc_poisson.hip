@@ -346,9 +346,9 @@ template <int ME>
 __global__ __launch_bounds__(256) void k_div_fft_theta_fwd_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  const DivIn dv{d.f[TC_QT], d.f[TC_QR], d.f[TC_QZ], d.usdt};
-  theta_fwd_body<ME, true, true>(blockIdx.x, nullptr, dv, d.g, d.spec, d.pt, d.twt,
-                                 (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
+  const DivIn dv{glob(d.f[TC_QT]), glob(d.f[TC_QR]), glob(d.f[TC_QZ]), d.usdt};
+  theta_fwd_body<ME, true, true>(blockIdx.x, nullptr, dv, glob(d.g), glob(d.spec), d.pt,
+                                 glob(d.twt), (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
 }
 
 // c2r along theta for row pairs; the imaginary parts of ktheta = 0 and
@@ -417,8 +417,8 @@ template <int ME>
 __global__ __launch_bounds__(256) void k_fft_theta_inv_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  theta_inv_body<ME>(blockIdx.x, d.spec, d.f[TC_DPH], d.pt, d.twt, (long long)d.g.nrm * d.g.nzm,
-                     d.g.nth2, d.g.sstr);
+  theta_inv_body<ME>(blockIdx.x, glob(d.spec), glob(d.f[TC_DPH]), d.pt, glob(d.twt),
+                     (long long)d.g.nrm * d.g.nzm, d.g.nth2, d.g.sstr);
 }
 
 // Workgroups are dispatched round-robin over the XCDs; hand each XCD a
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_fft_z_g(const StepDesc *__restrict__ tab
   const int inst = lin / ntiles, id = lin - inst * ntiles;
   const StepDesc &d = tab[inst];
   if (inst_stopped(d)) return;
-  fft_z_body<ME, NT>(id, d.spec, d.pz, d.twz, d.g.nth2, d.g.sstr, ntile, sg);
+  fft_z_body<ME, NT>(id, glob(d.spec), d.pz, glob(d.twz), d.g.nth2, d.g.sstr, ntile, sg);
 }
 
 // SolvePressureCorrection.F90:127-148: radial tridiagonal per (kz, ktheta)
@@ -748,7 +748,7 @@ template <int PQ>
 __global__ __launch_bounds__(512) void k_poisson_tridiag_g(const StepDesc *__restrict__ tab) {
   const StepDesc &d = tab[blockIdx.y];
   if (inst_stopped(d)) return;
-  poisson_body<PQ>(blockIdx.x, d.g, d.spec, d.inv_norm);
+  poisson_body<PQ>(blockIdx.x, glob(d.g), glob(d.spec), d.inv_norm);
 }
 
 // Launch shape of k_poisson_tridiag for a line of n rows: PT partitions of at

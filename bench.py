@@ -295,22 +295,26 @@ def sweep_leg(tc, steps_timed, plain_mpts):
     for p, seed in jobs:
         p = dict(p)
         prepared.append((p, sw.start_fields(tc, p, seed)))
-    shapes = sorted({(p["nthm"], p["nzm"], p["nrm"]) for p, _ in prepared})
-    tuning = {}
-    for sh in shapes:
-        p = next(q for q, _ in prepared if (q["nthm"], q["nzm"], q["nrm"]) == sh)
-        tuning[sh] = tc.autotune(p, steps=3)
-    print(f"[bench] sweep: {len(jobs)} instances, shapes {shapes}, tuning {tuning}",
-          file=sys.stderr, flush=True)
+    key = lambda p: (p["nthm"], p["nzm"], p["nrm"])
+    shapes = sorted({key(p) for p, _ in prepared})
+    # each side gets its own best schedule: the one-at-a-time jobs the
+    # lone-instance autotuner's, the batch the one timed per shape group
+    # (tuning time is left out of both sides)
+    solo = {sh: tc.autotune(next(q for q, _ in prepared if key(q) == sh), steps=3)
+            for sh in shapes}
+    tuning = sw.batch_autotune(tc, {sh: [q for q, _ in prepared if key(q) == sh] for sh in shapes},
+                               steps=3, base=solo)
+    print(f"[bench] sweep: {len(jobs)} instances, shapes {shapes}, tuning alone {solo}, "
+          f"batched {tuning}", file=sys.stderr, flush=True)
     # one job at a time on the plain path, per shape
     alone = {}
     for sh in shapes:
-        p, f = next((q, f) for q, f in prepared if (q["nthm"], q["nzm"], q["nrm"]) == sh)
-        alone[sh] = _alone_step_ms(tc, p, f)
+        p, f = next((q, f) for q, f in prepared if key(q) == sh)
+        alone[sh] = _alone_step_ms(tc, p, f, tuning=solo[sh])
     t0 = time.perf_counter()
     b = tc.Batch()
     for p, f in prepared:
-        iid = b.add(p, tuning[(p["nthm"], p["nzm"], p["nrm"])])
+        iid = b.add(p, tuning[key(p)])
         sw.load_and_project(b.solver(iid), f)
     b.groups()
     for i in range(len(jobs)):
@@ -329,7 +333,6 @@ def sweep_leg(tc, steps_timed, plain_mpts):
     cells = sum(p["nthm"] * p["nzm"] * p["nrm"] for p, _ in jobs)
     steps_job = jobs[0][0]["ntst"] + 1
     ms = 1e3 * el / steps_timed
-    key = lambda p: (p["nthm"], p["nzm"], p["nrm"])
     alone_steps_s = sum(alone[key(p)][0] * steps_job / 1e3 for p, _ in jobs)
     alone_setup_s = sum(alone[key(p)][1] for p, _ in jobs)
     batch_steps_s = steps_job * ms / 1e3
@@ -349,6 +352,9 @@ def sweep_leg(tc, steps_timed, plain_mpts):
         "setup_s": {"batched": round(setup_batch, 2), "one_at_a_time": round(alone_setup_s, 2)},
         "speedup_bound_at_plain_512_rate": round(alone_steps_s / bound_s, 2),
         "one_at_a_time_ms_per_step": {f"{a}x{b}x{c}": round(v[0], 3) for (a, b, c), v in alone.items()},
+        "tuning": {f"{a}x{b}x{c}": {"alone": {k: solo[(a, b, c)][k] for k in ("helm_lpt", "defer")},
+                                     "batched": {k: tuning[(a, b, c)][k] for k in ("helm_lpt", "defer")}}
+                   for (a, b, c) in shapes},
     }
 
 

@@ -89,6 +89,10 @@ typedef struct tc_tuning {
   int fft_pad;    /* LDS padding of the FFT tiles: one slot per 8 points (1, default),
                      per 4 (2) or per 16 (3); -1 off */
   int poisson_pq; /* rows per thread of the radial Poisson solve: 13 (default) or 26 */
+  int defer;      /* RK3 with even nthm: apply the pressure correction of substeps 1 and 2
+                     inside the next substep's Helmholtz kernel: 2 always, -1 never,
+                     1 from 8 Mi grid points up (default; TC_DEFER_CORRECTION 2/0/1).
+                     Changes results at the rounding level only. */
 } tc_tuning;
 
 /* tc_create with an explicit schedule; tc_set_tuning re-plans an existing

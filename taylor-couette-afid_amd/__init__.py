@@ -72,7 +72,8 @@ STATS = ["vt_mean", "vr_mean", "vz_mean", "wt_mean", "wr_mean", "wz_mean",
          "vt_rms", "vr_rms", "vz_rms", "wt_rms", "wr_rms", "wz_rms",
          "vtvr_mean", "vtvz_mean", "vrvz_mean", "pr_mean", "pr_rms", "disste"]
 
-TUNING_FIELDS = ["helm_lpt", "fft_t_lds", "fft_z_lds", "fft_z_nt", "fft_pad", "poisson_pq"]
+TUNING_FIELDS = ["helm_lpt", "fft_t_lds", "fft_z_lds", "fft_z_nt", "fft_pad", "poisson_pq",
+                 "defer"]
 
 
 class Params(ctypes.Structure):

@@ -314,6 +314,10 @@ tc_tuning resolve_tuning(const tc_tuning *in) {
     t.fft_pad = (v == 0) ? -1 : std::max(1, std::min(3, v));
   }
   if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", 13) == 26 ? 26 : 13;
+  if (t.defer == 0) {
+    const int v = env("TC_DEFER_CORRECTION", 1);
+    t.defer = v == 0 ? -1 : (v == 2 ? 2 : 1);
+  }
   return t;
 }
 
@@ -325,6 +329,8 @@ std::string apply_plans(tc_ctx *c, const tc_tuning &t) {
     return "poisson_pq must be 13 or 26 (got " + std::to_string(t.poisson_pq) + ")";
   if (t.helm_lpt < 0 || t.helm_lpt > 2)
     return "helm_lpt must be 0, 1 or 2 (got " + std::to_string(t.helm_lpt) + ")";
+  if (t.defer != -1 && t.defer != 1 && t.defer != 2)
+    return "defer must be 0, 1, 2 or -1 (got " + std::to_string(t.defer) + ")";
   tc::FFTPlan pt, pz;
   std::string e = tc::make_fft_plan(c->p.nthm, pt, t.fft_t_lds, 256);
   if (e.empty()) e = tc::make_fft_plan(c->p.nzm, pz, t.fft_z_lds, t.fft_z_nt == 256 ? 256 : 512);
@@ -355,6 +361,21 @@ void torque_from_sums(const tc_ctx *c, double slow, double sup, double *inner, d
 }  // namespace
 
 namespace tc {
+// tc_tuning::defer: 1 defers from 8 Mi grid points up, where the saved
+// passes pay; below that a lone grid lives in the caches and the
+// correction's extra instructions cost more than they save (a batch of many
+// small instances is another matter).  2: always; -1: never.  The spare
+// pressure array is allocated where the deferral applies.
+int set_deferral(tc_ctx *c, int mode) {
+  c->defer_corr = mode == 2 || (mode == 1 && (long long)c->nthm * c->nzm * c->nrm >= (8LL << 20));
+  if (c->defer_corr && c->p.nsst == 3 && c->nthm % 2 == 0 && !c->pr_alt) {
+    int rc = dev_alloc(c, (void **)&c->pr_alt, sizeof(double) * (size_t)c->N);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->pr_alt, 0, sizeof(double) * (size_t)c->N, c->stream));
+  }
+  return 0;
+}
+
 bool defers_correction(const tc_ctx *c) {
   return c->defer_corr && c->pr_alt && c->p.nsst == 3 && c->nthm % 2 == 0;
 }
@@ -465,19 +486,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
     TRY(dev_alloc(c, (void **)&c->f[q], sizeof(double) * (size_t)c->N));
     HTRY(hipMemsetAsync(c->f[q], 0, sizeof(double) * (size_t)c->N, c->stream));
   }
-  {
-    // 1 (default): defer from 8 Mi grid points up, where the saved passes
-    // pay; below that the fields live in the caches and the correction's
-    // extra instructions cost more than they save.  2: always; 0: never.
-    const char *e = std::getenv("TC_DEFER_CORRECTION");
-    const int mode = e ? std::atoi(e) : 1;
-    c->defer_corr = mode == 2 || (mode == 1 && (long long)c->nthm * c->nzm * c->nrm >= (8LL << 20));
-  }
-  // the spare pressure array, only where the deferred correction applies
-  if (c->defer_corr && c->p.nsst == 3 && c->nthm % 2 == 0) {
-    TRY(dev_alloc(c, (void **)&c->pr_alt, sizeof(double) * (size_t)c->N));
-    HTRY(hipMemsetAsync(c->pr_alt, 0, sizeof(double) * (size_t)c->N, c->stream));
-  }
+  TRY(tc::set_deferral(c, t.defer));
   const size_t M = (size_t)c->nzm * c->sstr;
   TRY(dev_alloc(c, (void **)&c->spec, sizeof(double2) * M * c->nrm));
   // grid vectors
@@ -620,7 +629,7 @@ int tc_set_tuning(tc_ctx *c, const tc_tuning *t) {
   if (!e.empty()) return fail(e);
   c->helm_lpt = r.helm_lpt ? r.helm_lpt : tc::choose_helm_lpt(c);
   c->poisson_pq = r.poisson_pq;
-  return 0;
+  return tc::set_deferral(c, r.defer);
 }
 
 int tc_get_tuning(tc_ctx *c, tc_tuning *t) {
@@ -692,7 +701,8 @@ int tc_autotune(const tc_params *p, int steps, tc_tuning *best) {
                         {&tc_tuning::fft_t_lds, {16384, 32768, 65536}},
                         {&tc_tuning::fft_z_lds, {32768, 65536}},
                         {&tc_tuning::fft_z_nt, {256, 512}},
-                        {&tc_tuning::fft_pad, {1, -1}}};
+                        {&tc_tuning::fft_pad, {1, -1}},
+                        {&tc_tuning::defer, {2, -1}}};
   for (const Knob &k : knobs) {
     if (rc) break;
     for (int v : k.values) {

@@ -273,6 +273,7 @@ void launch_triad(tc_ctx *c, int a, int b, int cc, double s);
 // group, for the shape and the tuning); tab: device table of n StepDesc of
 // the substep, dyn/res: device arrays of n
 bool defers_correction(const tc_ctx *c);
+int set_deferral(tc_ctx *c, int mode);
 void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
                               hipStream_t s, bool corr);
 void launch_correct_both_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t s);

@@ -148,18 +148,56 @@ def start_fields(tc, p, seed, amp=1e-2, cfl=0.8):
     return qt, qr, qz
 
 
-def footprint_bytes(p):
+def footprint_bytes(p, tuning=None):
     """Device bytes of one instance: eleven fields of nr planes (twelve where
-    the RK3 step defers its pressure correction: even nthm and, by default,
-    at least 8 Mi grid points; TC_DEFER_CORRECTION=2 always, 0 never), the
-    spectrum, and the small per-instance tables."""
+    the RK3 step defers its pressure correction: even nthm and the tuning's
+    ``defer`` 2, or 1 (the default, unless TC_DEFER_CORRECTION says 2 or 0)
+    with at least 8 Mi grid points), the spectrum, and the small
+    per-instance tables."""
     nthm, nzm, nrm = p["nthm"], p["nzm"], p["nrm"]
     sstr = ((nthm // 2 + 1) + 7) // 8 * 8
-    mode = os.environ.get("TC_DEFER_CORRECTION", "1")
+    mode = int((tuning or {}).get("defer", 0))
+    if mode == 0:
+        mode = {"0": -1, "2": 2}.get(os.environ.get("TC_DEFER_CORRECTION", "1"), 1)
     defer = (nthm % 2 == 0 and p.get("nsst", 3) == 3
-             and (mode == "2" or (mode == "1" and nthm * nzm * nrm >= 8 << 20)))
+             and (mode == 2 or (mode == 1 and nthm * nzm * nrm >= 8 << 20)))
     nf = 12 if defer else 11
     return 8 * nf * (nrm + 1) * nzm * nthm + 16 * nrm * nzm * sstr + (8 << 20)
+
+
+def batch_autotune(tc, groups, steps=3, base=None):
+    """Per-shape tuning for a Batch.  ``tc.autotune`` times one instance
+    alone; in a batch the grouped launches see all instances of a shape at
+    once, so both the lines per thread of the Helmholtz kernel (resident
+    blocks against tiles) and the deferred correction (which a lone small
+    grid, living in the caches, does not repay) can come out the other way.
+    For each shape, a batch of that shape's instances (``groups``: shape ->
+    list of params; zero fields, the real count) is timed over ``steps``
+    graph-replayed steps for each candidate: the lone-instance tuning
+    (``base``, else ``tc.autotune``) with helm_lpt 1 and 2, each with the
+    correction deferred by size and always.  Returns shape -> tuning."""
+    out = {}
+    for sh, plist in groups.items():
+        t0 = dict((base or {}).get(sh) or tc.autotune(plist[0], steps=steps))
+        cands = []
+        for lpt in (1, 2):
+            for defer in (t0.get("defer", 1), 2):
+                c = dict(t0, helm_lpt=lpt, defer=defer)
+                if c not in cands:
+                    cands.append(c)
+        best, best_ms = t0, None
+        for c in cands:
+            with tc.Batch() as b:
+                for p in plist:
+                    b.add(p, c)
+                b.advance(1, graph=True, sync_every=1)
+                t = time.perf_counter()
+                b.advance(steps, graph=True, sync_every=steps)
+                ms = (time.perf_counter() - t) / steps * 1e3
+            if best_ms is None or ms < 0.98 * best_ms:
+                best, best_ms = c, ms
+        out[sh] = best
+    return out
 
 
 def load_and_project(solver, fields):
@@ -312,7 +350,9 @@ class Server:
                 r.update(done=time.perf_counter() - self.t0, steps=steps, code=code)
                 if fields is not None:
                     r["fields"] = fields
-                self.used -= footprint_bytes(self.jobs[q][1])
+                p0 = self.jobs[q][1]
+                self.used -= footprint_bytes(
+                    p0, self.tuning.get((p0["nthm"], p0["nzm"], p0["nrm"])))
                 self.cells -= r["cells"]
                 self.nactive -= 1
             self.events += 1
@@ -357,7 +397,8 @@ class Server:
                 with self.cv:
                     for q in arrived:
                         p0 = jobs[q][1]
-                        need = footprint_bytes(p0)
+                        need = footprint_bytes(
+                            p0, self.tuning.get((p0["nthm"], p0["nzm"], p0["nrm"])))
                         c = p0["nthm"] * p0["nzm"] * p0["nrm"]
                         if self.nactive >= self.max_slots:
                             break

@@ -50,8 +50,8 @@ def _mix(tc):
     ]
 
 
-def _single(tc, p, st, nsteps):
-    s = tc.Solver(p)
+def _single(tc, p, st, nsteps, tuning=None):
+    s = tc.Solver(p, tuning=tuning)
     _load(s, st)
     diags = []
     for n in range(nsteps):
@@ -62,23 +62,30 @@ def _single(tc, p, st, nsteps):
     return out, diags
 
 
-@pytest.mark.parametrize("defer", ["1", "2"])  # default (off at these sizes), forced
+# default (off at these sizes), forced by the environment, forced per
+# instance by the tuning
+@pytest.mark.parametrize("defer", ["1", "2", "tuning"])
 def test_batch_matches_single_instance_bitwise(tc, monkeypatch, defer):
-    monkeypatch.setenv("TC_DEFER_CORRECTION", defer)
+    tuning = None
+    if defer == "tuning":
+        tuning = {"defer": 2}
+    else:
+        monkeypatch.setenv("TC_DEFER_CORRECTION", defer)
     nsteps = 5
     mix = _mix(tc)
     starts = [_start(tc, p, seed) for p, seed in mix]
     with tc.Batch() as b:
         ids = []
         for (p, _), st in zip(mix, starts):
-            i = b.add(p)
+            i = b.add(p, tuning)
             _load(b.solver(i), st)
             ids.append(i)
         ng, launches = b.groups()
         assert ng == 3, ng
         assert b.advance(nsteps, graph=True, sync_every=2) == len(mix)
         for (p, _), st, i in zip(mix, starts, ids):
-            ref, diags = _single(tc, p, st, nsteps)
+            assert b.solver(i).tuning()["defer"] == (2 if defer != "1" else 1)
+            ref, diags = _single(tc, p, st, nsteps, tuning)
             got = b.solver(i)
             for f in FIELDS:
                 a = got.get(f)
@@ -221,7 +228,7 @@ def test_autotune_schedule_runs_and_matches_default(tc):
 
 def test_tuning_values_outside_the_documented_sets_are_rejected(tc):
     p = _params(tc, 32, 24, 17)
-    for bad in ({"poisson_pq": 7}, {"helm_lpt": 3}):
+    for bad in ({"poisson_pq": 7}, {"helm_lpt": 3}, {"defer": 3}):
         with pytest.raises(tc.TCError):
             tc.Solver(p, tuning=bad)
     s = tc.Solver(p)

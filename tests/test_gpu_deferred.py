@@ -1,6 +1,7 @@
 """The fused RK3 step applies the pressure correction of substeps 1 and 2
-inside the next substep's Helmholtz kernel (TC_DEFER_CORRECTION: by default
-from 8 Mi grid points up, 2 at any size, 0 never).  Forced on these small
+inside the next substep's Helmholtz kernel (tuning ``defer`` 1 or
+TC_DEFER_CORRECTION 1: from 8 Mi grid points up, the default; 2 at any size;
+-1, or TC_DEFER_CORRECTION 0: never).  Forced on these small
 grids, it agrees with the per-substep schedule (one correction pass per
 substep) to rounding, with the tolerance below; where the deferral does not
 apply (odd nthm, or the default below the size threshold) the settings run
@@ -57,3 +58,51 @@ def test_default_keeps_the_per_substep_schedule_on_small_grids(tc, monkeypatch):
     _, _, b = _run(tc, monkeypatch, p, 0, 3)
     for f in FIELDS:
         assert np.array_equal(a[f], b[f]), f
+
+
+def _run_tuned(tc, p, tuning, nsteps, switch=None):
+    s = tc.Solver(p, tuning=tuning)
+    qt, qr, qz = tc.init_perturbed_couette(p, 33, 1e-2)
+    s.set("qt", qt); s.set("qr", qr); s.set("qz", qz)
+    s.project()
+    grew = 0
+    for n in range(nsteps):
+        if switch and n == switch[0]:
+            before = s.device_bytes()
+            s.set_tuning(switch[1])
+            grew = s.device_bytes() - before
+        s.step(n)
+    out = {f: s.get(f) for f in FIELDS}
+    t = s.tuning()
+    s.close()
+    return out, dict(t, grew=grew)
+
+
+def test_tuning_knob_is_the_environment_setting_per_instance(tc, monkeypatch):
+    p = small_params(tc, 32, 24, 33, ntst=1000, tout=1e-9)
+    _, _, env2 = _run(tc, monkeypatch, p, 2, 3)
+    _, _, env0 = _run(tc, monkeypatch, p, 0, 3)
+    monkeypatch.delenv("TC_DEFER_CORRECTION")
+    always, t2 = _run_tuned(tc, p, {"defer": 2}, 3)
+    never, tn = _run_tuned(tc, p, {"defer": -1}, 3)
+    default, t1 = _run_tuned(tc, p, None, 3)
+    assert (t2["defer"], tn["defer"], t1["defer"]) == (2, -1, 1)
+    for f in FIELDS:
+        assert np.array_equal(always[f], env2[f]), f
+        assert np.array_equal(never[f], env0[f]), f
+        assert np.array_equal(default[f], env0[f]), f  # off below 8 Mi points
+
+
+def test_deferral_switched_on_between_steps(tc, monkeypatch):
+    """tc_set_tuning turns the deferral on for a running context (allocating
+    the spare pressure array); every step ends with its fields in place, so
+    the switch needs no other change of state."""
+    monkeypatch.delenv("TC_DEFER_CORRECTION", raising=False)
+    p = small_params(tc, 32, 24, 33, ntst=1000, tout=1e-9)
+    switched, t = _run_tuned(tc, p, {"defer": -1}, 6, switch=(3, {"defer": 2}))
+    plain, _ = _run_tuned(tc, p, {"defer": -1}, 6)
+    # the spare pressure array (nr planes) appeared with the switch
+    assert t["defer"] == 2 and t["grew"] == 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"], t
+    for f in FIELDS:
+        err = np.linalg.norm(switched[f] - plain[f]) / np.linalg.norm(plain[f])
+        assert err < 1e-13, (f, err)

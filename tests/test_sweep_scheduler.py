@@ -173,3 +173,48 @@ def test_admission_order_holds_when_jobs_finish_during_a_pass(tc):
         assert adm == sorted(adm), rec
         for q in (1, 2):
             assert rec[q]["admitted"] >= rec[q - 1]["done"], rec
+
+
+def test_batch_autotune_times_each_candidate_on_the_whole_group(tc):
+    """batch_autotune builds one batch per candidate with all of a shape's
+    instances and keeps the fastest (ties to the lone-instance tuning)."""
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    seen = []
+
+    class TimedBatch(FakeBatch):
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            pass
+
+        def add(self, p, tuning=None):
+            seen.append((p["nthm"], dict(tuning)))
+            i = self.next
+            self.next += 1
+            self.inst[i] = dict(p=p, steps=0, code=0, tuning=tuning)
+            return i
+
+        def advance(self, nsteps, graph=True, sync_every=16):
+            # shape 32: lpt 2 with deferral is fastest; shape 16: nothing
+            # beats the lone-instance tuning by 2%
+            t = 0.0
+            for s in self.inst.values():
+                tu = s["tuning"]
+                if s["p"]["nthm"] == 32:
+                    t += 0.004 if (tu["helm_lpt"], tu["defer"]) == (2, 2) else 0.008
+                else:
+                    t += 0.004
+            time.sleep(t * nsteps)
+            return len(self.inst)
+
+    ftc = FakeTC(tc)
+    ftc.Batch = TimedBatch
+    base = {"helm_lpt": 1, "defer": 1, "poisson_pq": 13}
+    groups = {(32, 16, 9): [{"nthm": 32}] * 3, (16, 16, 9): [{"nthm": 16}] * 2}
+    out = sw.batch_autotune(ftc, groups, steps=2, base={(32, 16, 9): base, (16, 16, 9): base})
+    assert out[(32, 16, 9)] == dict(base, helm_lpt=2, defer=2)
+    assert out[(16, 16, 9)] == base
+    # four candidates per shape, each with the shape's whole group
+    assert sum(1 for n, _ in seen if n == 32) == 4 * 3
+    assert sum(1 for n, _ in seen if n == 16) == 4 * 2

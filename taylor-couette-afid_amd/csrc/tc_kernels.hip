@@ -553,10 +553,20 @@ struct CorrIn {
   double dt, al, be;  // al of the substep that produced dph
 };
 
+// CARRY: plane k-1 is not read from the ring but from values the thread
+// kept from the previous plane.  The explicit terms and right-hand sides
+// read plane k-1 only at (dj, di) = (0, 0), (0, +1) and (+1, 0): m00, m01
+// and m10 hold those (the centre, theta+1 and z+1 values of the previous
+// iteration's plane k).
+template <bool CARRY>
 struct LAcc {
   const double *m, *c, *p;  // planes k-1, k, k+1 of one field
   int base;                 // (row+1)*FT_W + tx+1
+  double m00, m01, m10;
   DEVI double operator()(int dk, int dj, int di) const {
+    if constexpr (CARRY) {
+      if (dk < 0) return dj != 0 ? m10 : (di != 0 ? m01 : m00);
+    }
     const double *s = dk < 0 ? m : (dk > 0 ? p : c);
     return s[base + dj * FT_W + di];
   }
@@ -669,6 +679,11 @@ DEVI void helm_fwd_body(
   long long l[LPT];
   int base[LPT];
   double fic[LPT], foc[LPT], h1[LPT], h2[LPT], h3[LPT], ct[LPT], cr[LPT], cz[LPT], pr_km[LPT];
+  // !CORR: plane k-1 values carried from the previous iteration (qt centre,
+  // qt theta+1, qr centre, qz centre, qz z+1), so that no thread reads the
+  // ring slot of plane k-1 and the staging of plane k+2 into it needs no
+  // barrier of its own
+  double cm[LPT][5];
   LUView Tt[LPT], Tz[LPT];
   const LUView Tr = lu_view(lur, nt);
 #pragma unroll
@@ -689,6 +704,8 @@ DEVI void helm_fwd_body(
     h2[u] = 0.0;
     h3[u] = (valid[u] && hist) ? ld_nt(ru3 + l[u]) : 0.0;
     ct[u] = cr[u] = cz[u] = pr_km[u] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) cm[u][q] = 0.0;
   }
   // planes 1 and 2 of the velocity, plane 1 of the pressure (CORR: planes 1
   // and 2 of dph first, then the corrected values)
@@ -769,10 +786,13 @@ DEVI void helm_fwd_body(
 #pragma unroll
     for (int u = 0; u < LPT; u++) {
       if (!valid[u]) continue;
-      LAcc T, R, Z;
+      LAcc<!CORR> T, R, Z;
       T.m = sq[0][sm]; T.c = sq[0][sc]; T.p = sq[0][sn]; T.base = base[u];
       R.m = sq[1][sm]; R.c = sq[1][sc]; R.p = sq[1][sn]; R.base = base[u];
       Z.m = sq[2][sm]; Z.c = sq[2][sc]; Z.p = sq[2][sn]; Z.base = base[u];
+      T.m00 = cm[u][0]; T.m01 = cm[u][1]; T.m10 = 0.0;
+      R.m00 = cm[u][2]; R.m01 = R.m10 = 0.0;
+      Z.m00 = cm[u][3]; Z.m01 = 0.0; Z.m10 = cm[u][4];
       const double pr_c = spk[base[u]], pr_im = spk[base[u] - 1], pr_jm = spk[base[u] - FT_W];
       const long long o = (long long)(k - 1) * P + l[u];
       // q_theta
@@ -810,6 +830,13 @@ DEVI void helm_fwd_body(
         }
       }
       pr_km[u] = pr_c;
+      if constexpr (!CORR) {
+        cm[u][0] = T(0, 0, 0);
+        cm[u][1] = T(0, 0, 1);
+        cm[u][2] = R(0, 0, 0);
+        cm[u][3] = Z(0, 0, 0);
+        cm[u][4] = Z(0, 1, 0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < LPT; u++) {
@@ -822,8 +849,10 @@ DEVI void helm_fwd_body(
 #pragma unroll
       for (int q = 0; q < NE2; q++)
         if (q < NE2 - 1 || has_last2) sd[(k + 2) % 3][threadIdx.x + q * 256] = nd[q];
+      // the ring slot of plane k-1 is read above, and the dph plane k+2 just
+      // written is read by the staging below
+      __syncthreads();
     }
-    __syncthreads();
     // in the last iteration these fill slots no later plane reads
     const int s2 = (k + 2) % 3;
 #pragma unroll

@@ -60,6 +60,7 @@ ABI_SYMBOLS = [
     "tc_create_tuned", "tc_set_tuning", "tc_get_tuning", "tc_autotune",
     "tc_batch_create", "tc_batch_destroy", "tc_batch_add", "tc_batch_remove", "tc_batch_ctx",
     "tc_batch_advance", "tc_batch_status", "tc_batch_groups",
+    "tc_batch_write_restart", "tc_batch_read_restart",
     "tc_write_restart", "tc_read_restart",
     "tc_calc_stats", "tc_calc_dissipation", "tc_get_stats", "tc_set_stats",
     "tc_last_dissipation", "tc_dump_rcut", "tc_dump_thcut", "tc_dump_streamwise_avg",
@@ -170,6 +171,8 @@ def lib():
     L.tc_batch_advance.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.tc_batch_status.argtypes = [_V, ctypes.c_int, _I, _I, _D]
     L.tc_batch_groups.argtypes = [_V, _I, _I]
+    L.tc_batch_write_restart.argtypes = [_V, ctypes.c_int, ctypes.c_char_p]
+    L.tc_batch_read_restart.argtypes = [_V, ctypes.c_int, ctypes.c_char_p, _I]
     L.tc_write_restart.argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
     L.tc_read_restart.argtypes = [_V, ctypes.c_char_p, _I]
     L.tc_calc_stats.argtypes = [_V]
@@ -551,6 +554,17 @@ class Batch:
         o = np.zeros(9)
         _check(lib().tc_batch_status(self.h, int(iid), ctypes.byref(steps), ctypes.byref(code), _dp(o)))
         return steps.value, code.value, dict(zip(self.STATUS, o))
+
+    def write_restart(self, iid, path):
+        """Whole-state checkpoint of one instance at its current step."""
+        _check(lib().tc_batch_write_restart(self.h, int(iid), os.fsencode(path)))
+
+    def read_restart(self, iid, path):
+        """Loads a checkpoint into an instance that has not stepped yet;
+        returns the step it continues from."""
+        n = ctypes.c_int()
+        _check(lib().tc_batch_read_restart(self.h, int(iid), os.fsencode(path), ctypes.byref(n)))
+        return n.value
 
     def groups(self):
         n, l = ctypes.c_int(), ctypes.c_int()

@@ -520,6 +520,45 @@ int tc_batch_status(tc_batch *b, int id, int *steps, int *code, double *out9) {
   return 0;
 }
 
+// One instance's whole state at its current step, in the tc_write_restart
+// format: the batch keeps each instance's step count and time itself, so
+// they are handed to the context first (the CFL cache is marked stale: a
+// run continued on the single-instance path evaluates it afresh).
+int tc_batch_write_restart(tc_batch *b, int id, const char *path) {
+  if (!b || !path || id < 0 || id >= (int)b->inst.size() || !b->inst[(size_t)id].live)
+    return tc::set_error("tc_batch_write_restart: no such instance");
+  int rc = harvest(b);
+  if (rc) return rc;
+  Inst &I = b->inst[(size_t)id];
+  tc_ctx *c = I.c;
+  c->time = I.time;
+  c->dt = c->p.dtmax;
+  c->cfl_valid = false;
+  return tc_write_restart(c, path, I.done ? I.end_step + 1 : I.steps);
+}
+
+// Loads a checkpoint into an instance that has not stepped yet; it then
+// continues from the checkpoint's step and time, bit for bit the run that
+// wrote it.  The checkpoint's dt must be the instance's fixed dt.
+int tc_batch_read_restart(tc_batch *b, int id, const char *path, int *ntime) {
+  if (!b || !path || id < 0 || id >= (int)b->inst.size() || !b->inst[(size_t)id].live)
+    return tc::set_error("tc_batch_read_restart: no such instance");
+  int rc = harvest(b);
+  if (rc) return rc;
+  Inst &I = b->inst[(size_t)id];
+  if (I.steps != 0 || I.done)
+    return tc::set_error("tc_batch_read_restart: the instance has already stepped");
+  int nt = 0;
+  rc = tc_read_restart(I.c, path, &nt);
+  if (rc) return rc;
+  if (I.c->dt != I.c->p.dtmax)
+    return tc::set_error("tc_batch_read_restart: the checkpoint's dt is not the instance's DTMAX");
+  I.steps = nt;
+  I.time = I.c->time;
+  if (ntime) *ntime = nt;
+  return 0;
+}
+
 int tc_batch_groups(tc_batch *b, int *ngroups, int *launches_per_step) {
   if (!b) return tc::set_error("null argument");
   if (b->dirty) {

@@ -217,17 +217,18 @@ class _ClassWorker:
         self.shape = shape
         self.b = server.tc.Batch()
         self.qlock = threading.Lock()
-        self.pending = []          # (job index, params, fields, tuning)
+        self.pending = []          # (job index, params, fields, tuning, checkpoint or None)
         self.active = {}           # batch id -> job index
+        self.saved = {}            # batch id -> step of its last checkpoint
         self.wake = threading.Event()
         self.stop = False
         self.error = None
         self.thread = threading.Thread(target=self._loop, daemon=True)
         self.thread.start()
 
-    def admit(self, q, p, fields, tuning):
+    def admit(self, q, p, fields, tuning, resume=None):
         with self.qlock:
-            self.pending.append((q, p, fields, tuning))
+            self.pending.append((q, p, fields, tuning, resume))
         self.wake.set()
 
     def _loop(self):
@@ -241,9 +242,14 @@ class _ClassWorker:
                     break
                 with self.qlock:
                     new, self.pending = self.pending, []
-                for q, p, fields, tuning in new:
+                for q, p, fields, tuning, resume in new:
                     iid = b.add(p, tuning)
-                    load_and_project(b.solver(iid), fields)
+                    if resume:
+                        self.saved[iid] = b.read_restart(iid, resume)
+                        srv._resumed(q, self.saved[iid])
+                    else:
+                        load_and_project(b.solver(iid), fields)
+                        self.saved[iid] = 0
                     self.active[iid] = q
                     srv._admitted(q)
                 del new
@@ -252,6 +258,10 @@ class _ClassWorker:
                 b.advance(srv.chunk, graph=srv.graph, sync_every=srv.chunk)
                 for iid in list(self.active):
                     steps, code, _ = b.status(iid)
+                    every = srv.checkpoint_every
+                    if every and (code or steps // every > self.saved[iid] // every):
+                        srv._checkpoint(b, iid, self.active[iid])
+                        self.saved[iid] = steps
                     if code:
                         q = self.active.pop(iid)
                         fields = None
@@ -259,6 +269,7 @@ class _ClassWorker:
                             s = b.solver(iid)
                             fields = tuple(s.get(f) for f in ("qt", "qr", "qz", "pr"))
                         b.remove(iid)
+                        del self.saved[iid]
                         srv._finished(q, steps, code, fields)
         except Exception as e:  # reported by Server.run
             self.error = e
@@ -319,9 +330,19 @@ class Server:
     each advanced by its own thread; admission is global."""
 
     def __init__(self, tc, hbm_budget=200e9, chunk=25, graph=True, tuning=None, workers=4,
-                 max_slots=16, max_cells=400e6, order="arrival"):
+                 max_slots=16, max_cells=400e6, order="arrival", checkpoint_dir=None,
+                 checkpoint_every=0):
         # graph=True: each class worker replays its batch's whole step as a
-        # hipGraph, captured again whenever jobs join or leave
+        # hipGraph, captured again whenever jobs join or leave.
+        # checkpoint_dir / checkpoint_every: each job's whole state is written
+        # to checkpoint_dir/job<index>.tcr (WriteFlowField's role) at the first
+        # chunk boundary past every multiple of checkpoint_every steps and
+        # when it ends; run(resume=True) continues each job that has one
+        # from it, bit for bit the run that wrote it.
+        if checkpoint_every and not checkpoint_dir:
+            raise ValueError("checkpoint_every needs a checkpoint_dir")
+        self.checkpoint_dir = checkpoint_dir
+        self.checkpoint_every = int(checkpoint_every)
         self.tc = tc
         self.budget = hbm_budget
         self.max_slots = max_slots
@@ -343,6 +364,19 @@ class Server:
         with self.cv:
             self.rec[q]["admitted"] = time.perf_counter() - self.t0
 
+    def _resumed(self, q, step):
+        with self.cv:
+            self.rec[q]["resumed_at"] = step
+
+    def checkpoint_path(self, q):
+        return os.path.join(self.checkpoint_dir, f"job{q:04d}.tcr")
+
+    def _checkpoint(self, b, iid, q):
+        # written whole before it replaces the previous one
+        path = self.checkpoint_path(q)
+        b.write_restart(iid, path + ".tmp")
+        os.replace(path + ".tmp", path)
+
     def _finished(self, q, steps, code, fields):
         with self.cv:
             if q is not None:
@@ -358,10 +392,15 @@ class Server:
             self.events += 1
             self.cv.notify_all()
 
-    def run(self, jobs, keep_fields=False):
+    def run(self, jobs, keep_fields=False, resume=False):
         """jobs: (arrival s, params, seed).  Returns one record per job:
         arrival, admitted, done (s since start), steps, code, cells; with
-        keep_fields, the final (qt, qr, qz, pr)."""
+        keep_fields, the final (qt, qr, qz, pr); with resume, a job whose
+        checkpoint exists continues from it (resumed_at: its step)."""
+        if resume and not self.checkpoint_dir:
+            raise ValueError("resume needs a checkpoint_dir")
+        if self.checkpoint_dir:
+            os.makedirs(self.checkpoint_dir, exist_ok=True)
         self.jobs, self.keep_fields = jobs, keep_fields
         self.rec = [None] * len(jobs)
         self.used, self.cells, self.nactive = 0.0, 0, 0
@@ -421,7 +460,10 @@ class Server:
                     with self.cv:
                         self.rec[q] = dict(arrival=jobs[q][0], admitted=None, cells=c,
                                            steps_total=p["ntst"] + 1)
-                    workers[sh].admit(q, p, fields, self.tuning.get(sh))
+                    ck = self.checkpoint_path(q) if resume else None
+                    if ck and not os.path.exists(ck):
+                        ck = None
+                    workers[sh].admit(q, p, fields, self.tuning.get(sh), ck)
                     del fields
                 # sleep until a job finishes or the next arrival; a job that
                 # finished during this pass wakes the loop at once

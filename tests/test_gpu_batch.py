@@ -236,3 +236,46 @@ def test_tuning_values_outside_the_documented_sets_are_rejected(tc):
         s.set_tuning({"poisson_pq": 20})
     assert s.tuning()["poisson_pq"] == 13
     s.close()
+
+
+def test_batch_checkpoint_continues_bit_for_bit(tc, tmp_path):
+    """tc_batch_write_restart mid-run, tc_batch_read_restart into a fresh
+    batch (and tc_read_restart on the single-instance path): both continue
+    as the uninterrupted batch does, bit for bit."""
+    mix = _mix(tc)[2:4]
+    starts = [_start(tc, p, seed) for p, seed in mix]
+    paths = [str(tmp_path / f"inst{q}.tcr") for q in range(len(mix))]
+    with tc.Batch() as a:
+        ids = []
+        for (p, _), st in zip(mix, starts):
+            i = a.add(p)
+            _load(a.solver(i), st)
+            ids.append(i)
+        a.advance(6, graph=True, sync_every=4)
+        for i, path in zip(ids, paths):
+            a.write_restart(i, path)
+        a.advance(5, graph=True, sync_every=4)
+        want = [{f: a.solver(i).get(f) for f in FIELDS} for i in ids]
+        assert all(a.status(i)[:2] == (11, 0) for i in ids)
+    with tc.Batch() as b:
+        ids = [b.add(p) for p, _ in mix]
+        assert [b.read_restart(i, path) for i, path in zip(ids, paths)] == [6, 6]
+        with pytest.raises(tc.TCError):
+            b.read_restart(ids[0], paths[0])  # already past step 0
+        extra = b.add(mix[0][0])
+        with pytest.raises(tc.TCError):
+            b.read_restart(extra, paths[1])  # another grid
+        b.remove(extra)
+        b.advance(5, graph=True, sync_every=2)
+        for i, w in zip(ids, want):
+            assert b.status(i)[:2] == (11, 0)
+            for f in FIELDS:
+                assert np.array_equal(b.solver(i).get(f), w[f]), (i, f)
+    for (p, _), path, w in zip(mix, paths, want):
+        s = tc.Solver(p)
+        assert s.read_restart(path) == 6
+        code, last, _ = s.run(6, 5)
+        assert (code, last) == (0, 10)
+        for f in FIELDS:
+            assert np.array_equal(s.get(f), w[f]), f
+        s.close()

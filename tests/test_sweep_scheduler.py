@@ -218,3 +218,41 @@ def test_batch_autotune_times_each_candidate_on_the_whole_group(tc):
     # four candidates per shape, each with the shape's whole group
     assert sum(1 for n, _ in seen if n == 32) == 4 * 3
     assert sum(1 for n, _ in seen if n == 16) == 4 * 2
+
+
+def test_checkpoints_follow_the_cadence_and_resume_continues(tc, tmp_path):
+    """checkpoint_every: a job's state is written at the first chunk boundary
+    past every multiple and when it ends (each write replacing the last);
+    run(resume=True) continues a job from its checkpoint's step."""
+    import json
+    import os
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    writes = []
+
+    class CkptBatch(FakeBatch):
+        def write_restart(self, i, path):
+            writes.append(self.inst[i]["steps"])
+            with open(path, "w") as f:
+                json.dump({"steps": self.inst[i]["steps"]}, f)
+
+        def read_restart(self, i, path):
+            with open(path) as f:
+                n = json.load(f)["steps"]
+            self.inst[i]["steps"] = n
+            return n
+
+    ftc = FakeTC(tc)
+    ftc.Batch = CkptBatch
+    g = (16, 16, 9)
+    jobs = _jobs(sw, [g], [0.0], [20])  # ntst 19: 20 steps
+    srv = sw.Server(ftc, chunk=4, workers=1, checkpoint_dir=str(tmp_path), checkpoint_every=6)
+    rec = srv.run(jobs)
+    assert rec[0]["code"] == 555 and rec[0]["steps"] == 20
+    assert writes == [8, 12, 20], writes  # past 6, past 12, the end
+    assert sorted(os.listdir(tmp_path)) == ["job0000.tcr"]
+    # the same job extended to 40 steps continues from step 20
+    longer = [(t, dict(p, ntst=39), s) for t, p, s in jobs]
+    rec = srv.run(longer, resume=True)
+    assert rec[0]["resumed_at"] == 20 and rec[0]["steps"] == 40 and rec[0]["code"] == 555
+    with pytest.raises(ValueError):
+        sw.Server(ftc, checkpoint_every=5)

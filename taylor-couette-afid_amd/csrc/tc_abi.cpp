@@ -298,7 +298,8 @@ void set_substep(tc_ctx *c, int ns) {
 
 // Effective tuning: explicit nonzero fields first, then the TC_* environment
 // overrides, then the defaults (which come from sweeps at 512x512x385).
-tc_tuning resolve_tuning(const tc_tuning *in) {
+// nrm: the radial size, for the defaults that depend on it
+tc_tuning resolve_tuning(const tc_tuning *in, int nrm) {
   tc_tuning t{};
   if (in) t = *in;
   auto env = [](const char *name, int dflt) {
@@ -313,7 +314,11 @@ tc_tuning resolve_tuning(const tc_tuning *in) {
     const int v = env("TC_FFT_PAD", 1);
     t.fft_pad = (v == 0) ? -1 : std::max(1, std::min(3, v));
   }
-  if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", 13) == 26 ? 26 : 13;
+  // rows per thread of the radial Poisson solve: 13 up to nrm = 512, 26
+  // beyond, where the 2*PT-row reduced system of 13-row partitions grows
+  // long (at 1024x1024x769: 4.0 ms per launch with 26 against 7.3 with 13;
+  // at 512x512x385 13 is the faster)
+  if (t.poisson_pq == 0) t.poisson_pq = env("TC_POISSON_PQ", nrm > 512 ? 26 : 13) == 26 ? 26 : 13;
   if (t.defer == 0) {
     const int v = env("TC_DEFER_CORRECTION", 1);
     t.defer = v == 0 ? -1 : (v == 2 ? 2 : 1);
@@ -443,7 +448,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
   if (!c) return fail("out of host memory");
   c->p = *p;
   std::string e = tc::build_grid(*p, c->g);
-  const tc_tuning t = resolve_tuning(tun);
+  const tc_tuning t = resolve_tuning(tun, p->nrm);
   if (e.empty()) e = apply_plans(c, t);
   if (!e.empty()) {
     delete c;
@@ -624,7 +629,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
 int tc_set_tuning(tc_ctx *c, const tc_tuning *t) {
   CTXCHK(c);
   HIPCHK(hipStreamSynchronize(c->stream));
-  const tc_tuning r = resolve_tuning(t);
+  const tc_tuning r = resolve_tuning(t, c->nrm);
   const std::string e = apply_plans(c, r);
   if (!e.empty()) return fail(e);
   c->helm_lpt = r.helm_lpt ? r.helm_lpt : tc::choose_helm_lpt(c);
@@ -1070,9 +1075,11 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
   double out[9] = {instcfl, c->dt, 0, 0, 0, 0, 0, 0, 0};
   const double tnew = c->time + c->dt;
   const bool logs = (ntime == 1 || std::fmod(tnew, p.tout) < c->dt);
-  // a logging iteration evaluates its diagnostics inside the last
-  // correction pass where that pass is the fused one
-  c->diag_in_correction = logs && c->fused && !c->batch_owned && tc::correct_diag_applies(c);
+  // where the last correction pass is the fused one it also evaluates the
+  // diagnostics: a logging iteration uses them all, any other keeps the
+  // CFL for the next iteration's dt rule (CalcMaxCFL of main.F90:156 on
+  // the same fields), which saves that iteration its own CFL pass
+  c->diag_in_correction = c->fused && !c->batch_owned && tc::correct_diag_applies(c);
   rc = tc_time_marcher(c);
   const bool diag_done = c->diag_in_correction;
   c->diag_in_correction = false;
@@ -1102,6 +1109,14 @@ int tc_step(tc_ctx *c, int ntime, double walltime, double *o) {
       rc = run_stats(c, true, true);
       if (rc) return rc;
     }
+  } else if (diag_done) {
+    rc = launch_status();
+    if (rc) return rc;
+    double v[10];
+    rc = read_reduction(c, 10, v);
+    if (rc) return rc;
+    c->cfl_cached = std::max(0.00000001, v[0]);
+    c->cfl_valid = true;
   }
   if (c->time > p.tmax) errorcode = 333;
   if (walltime > p.walltimemax) errorcode = 334;

@@ -1003,32 +1003,25 @@ static void launch_fwd(tc_ctx *c, const double *pr, const CorrIn &ci, double *pr
                      lu_slot(c, LU_QZ), c->lu_nt, ntx, ntz, ci, c->f[TC_DPH], pr_out);
 }
 
-// Lines per thread of the fused kernel: the variant that needs the fewest
-// rounds of resident blocks (one line per thread on a tie).
+// Lines per thread of the fused kernel: two where the two-line tiles alone
+// fill every resident slot of the device at least once, one otherwise.
+// Two lines per thread give each block twice the work per plane for less
+// than twice the time, but half the tiles: at 512x512x385 the 512 two-line
+// tiles fill the 512 slots once (2771 us per launch against 3445 for one
+// line), at 1024x1024x769 four times (22.4 ms against 26.0), while a lone
+// 256^2x193 grid (128 two-line tiles) runs faster with one line.
 int choose_helm_lpt(const tc_ctx *c) {
   int dev = 0, cus = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus <= 0) return 1;
-  double best = 1e30;
-  int pick = 1;
-  for (int lpt = 1; lpt <= 2; lpt++) {
-    int per_cu = 0;
-    hipError_t e = (lpt == 1)
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 1, false>, 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 2, false>, 256, 0);
-    if (e != hipSuccess || per_cu <= 0) continue;
-    const long long tiles = (long long)((c->nthm + FT_I - 1) / FT_I) * ((c->nzm + 4 * lpt - 1) / (4 * lpt));
-    const long long slots = (long long)per_cu * cus;
-    const long long rounds = (tiles + slots - 1) / slots;
-    // each round costs one march; a block with two lines per thread works ~1.6x longer
-    const double cost = (double)rounds * (lpt == 1 ? 1.0 : 1.6);
-    if (cost < best - 1e-9) {
-      best = cost;
-      pick = lpt;
-    }
-  }
-  return pick;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_helm_fwd<true, 2, true>, 256, 0) !=
+          hipSuccess ||
+      per_cu <= 0)
+    return 1;
+  const long long tiles = (long long)((c->nthm + FT_I - 1) / FT_I) * ((c->nzm + 7) / 8);
+  return tiles >= (long long)per_cu * cus ? 2 : 1;
 }
 
 void launch_helmholtz_fused(tc_ctx *c) {

@@ -100,6 +100,19 @@ def test_solve_pressure_correction(tc, oracle, shape):
     _cmp(dev, orc, "dph", slice(0, nrm), tol=1e-11)
 
 
+# 26 rows per thread (the default beyond nrm = 512, where 13-row partitions
+# make the reduced system long) and 13, on short and long radial lines
+@pytest.mark.parametrize("pq", [13, 26])
+@pytest.mark.parametrize("shape", [(32, 24, 33), (16, 8, 600), (24, 16, 769)])
+def test_solve_pressure_correction_rows_per_thread(tc, oracle, shape, pq):
+    p, dev, orc = _setup(tc, oracle, shape, 43)
+    nrm = shape[2]
+    assert dev.tuning()["poisson_pq"] == (26 if nrm > 512 else 13)
+    dev.set_tuning({"poisson_pq": pq})
+    dev.SolvePressureCorrection(); orc.call("solve_pressure_correction")
+    _cmp(dev, orc, "dph", slice(0, nrm), tol=1e-11)
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_reductions(tc, oracle, shape):
     p, dev, orc = _setup(tc, oracle, shape, 51, masks=True)

@@ -45,6 +45,9 @@ PROFILE_COMMANDS = {
     "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE --output-format csv --kernel-include-regex "
                   "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
                   f"-d OUT/WRITE_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 {_LEG}",
+    # the batched sweep: 2 untimed + 6 timed graph-replayed steps of the 48-instance mix
+    "sweep": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/sweep -o sweep "
+             "-- python3 bench.py --steps 1 --warmup 1 --no-cpu --sweep-steps 6 --served-jobs 0",
 }
 
 
@@ -487,6 +490,51 @@ def summarize_profile(out_dir, workload, p):
     open(traffic_path, "w").write("\n".join(lines) + "\n")
     print(stats_path)
     print(traffic_path)
+    sweep_csv = os.path.join(out_dir, "sweep", "sweep_kernel_trace.csv")
+    if os.path.exists(sweep_csv):
+        print(summarize_sweep(sweep_csv))
+
+
+def summarize_sweep(path):
+    """Grouped kernels of the batched sweep's own steps: the dispatches after
+    the last single-instance kernel (autotuning, the one-at-a-time timings
+    and the projections come first), i.e. the 8 graph-replayed steps of the
+    48-instance mix, with the four shape groups on their own streams."""
+    import csv
+    import collections
+
+    rows = list(csv.DictReader(open(path)))
+    grouped = lambda r: "_g" in _kernel_short(r["Kernel_Name"]).split("<")[0]
+    cut = max(int(r["End_Timestamp"]) for r in rows
+              if not grouped(r) and "rocclr" not in r["Kernel_Name"])
+    g = [r for r in rows if grouped(r) and int(r["Start_Timestamp"]) > cut]
+    t0 = min(int(r["Start_Timestamp"]) for r in g)
+    t1 = max(int(r["End_Timestamp"]) for r in g)
+    busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in g)
+    agg = collections.defaultdict(list)
+    for r in g:
+        agg[(_kernel_short(r["Kernel_Name"]), int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]))].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    steps = 8
+    lines = ["# Grouped kernels of the batched sweep (48 instances, 4 shape groups)", "",
+             "Command (from /tmp, TMPDIR=/tmp, OUT = an output folder, repository root as",
+             "the working directory of bench.py):", "", "    " + PROFILE_COMMANDS["sweep"], "",
+             f"The batch's own {steps} steps (2 untimed + 6 timed): the grouped dispatches after "
+             "the last", "single-instance kernel.  The four shape groups run on their own "
+             "streams inside one", "hipGraph per step, so their kernels overlap:", "",
+             f"- wall span {(t1 - t0) / 1e6:.1f} ms, {(t1 - t0) / 1e6 / steps:.2f} ms per step;",
+             f"- kernel time summed over dispatches {busy / 1e6:.1f} ms "
+             f"({busy / (t1 - t0):.2f} kernels in flight on average);",
+             f"- {len(g) // steps} grouped dispatches per step.", "",
+             "| kernel | grid x (threads) | grid y (instances, or planes x instances) | calls | "
+             "average us | total ms | share of summed % |",
+             "|---|---|---|---|---|---|---|"]
+    for (k, gx, gy), v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"| {k} | {gx} | {gy} | {len(v)} | {sum(v) / len(v):.1f} | {sum(v) / 1e3:.2f} | "
+                     f"{100 * sum(v) * 1e3 / busy:.1f} |")
+    out = os.path.join(PROFILES, "grouped_kernels_sweep48_stats.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    return out
 
 
 def traffic_of(workload, timer):

@@ -1689,7 +1689,7 @@ DEVI void correct_diag_body(const int bx, const int by, const int seg, const int
                             const unsigned char *__restrict__ icyl,
                             const unsigned char *__restrict__ ocyl, double virotint,
                             double virotext, double del, double deln, double pi,
-                            double *__restrict__ part, int blk, int sync) {
+                            double *__restrict__ part, int blk) {
 #pragma clang fp contract(off)
   double v[10] = {0.0, 0.0, -100.0, -100.0, -100.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int lane = threadIdx.x;
@@ -1811,10 +1811,6 @@ DEVI void correct_diag_body(const int bx, const int by, const int seg, const int
     dkm = d;
     d = dkp;
     rraw = rnext;
-    // every sync planes the block's four rows meet, so that each row's dph
-    // and qz neighbours are read while the wave that owns them has them in
-    // cache (many planes apart they are gone from L2)
-    if (sync > 0 && (kc - k0) % sync == sync - 1) __syncthreads();
   }
   const int ops[10] = {1, 1, 1, 1, 1, 0, 0, 0, 0, 0};
   block_reduce<10>(v, ops, part + 10 * blk);
@@ -1826,14 +1822,17 @@ __global__ __launch_bounds__(256) void k_correct_diag(
     double *__restrict__ rout, double *__restrict__ zout, const double *__restrict__ dph,
     double dt, double al, double be, const unsigned char *__restrict__ icyl,
     const unsigned char *__restrict__ ocyl, double virotint, double virotext, double del,
-    double deln, double pi, double *__restrict__ part, int sync, int xcdmap) {
+    double deln, double pi, double *__restrict__ part) {
   // one-dimensional grid of (theta block, z block, r segment) tiles, theta
   // fastest, handed out XCD by XCD so that z-neighbouring tiles share an L2
+  // (the rows of a block run unsynchronised: meeting every few planes, so
+  // that neighbours' dph and qz are read while still cached, measured no
+  // faster)
   const int nbx = (g.nthm / 2 + 63) / 64, nby = (g.nzm + 3) / 4;
-  const int blk = xcdmap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = blk % nbx, by = (blk / nbx) % nby, seg = blk / (nbx * nby);
   correct_diag_body(bx, by, seg, gridDim.x / (nbx * nby), g, qt, qr, qz, pr, tout, rout, zout, dph,
-                    dt, al, be, icyl, ocyl, virotint, virotext, del, deln, pi, part, blk, sync);
+                    dt, al, be, icyl, ocyl, virotint, virotext, del, deln, pi, part, blk);
 }
 
 // Segments of r per column: enough blocks for about six per CU (at
@@ -1853,23 +1852,6 @@ int correct_diag_segments(const tc_ctx *c) {
   return (int)std::max<long long>(s, 1);
 }
 
-// Planes between the meetings of a block's rows (TC_CDIAG_SYNC; 0: never),
-// and whether tiles go out XCD by XCD (TC_CDIAG_XCD; default 1).
-static int correct_diag_sync() {
-  static const int v = [] {
-    const char *e = std::getenv("TC_CDIAG_SYNC");
-    return e ? std::atoi(e) : 0;
-  }();
-  return v;
-}
-static int correct_diag_xcd() {
-  static const int v = [] {
-    const char *e = std::getenv("TC_CDIAG_XCD");
-    return e ? std::atoi(e) : 1;
-  }();
-  return v;
-}
-
 bool correct_diag_applies(const tc_ctx *c) { return c->nthm % 2 == 0 && c->nrm >= 3; }
 
 // The corrected qt, qr, qz land in the spare arrays rhs, dq, qcap (free at
@@ -1883,8 +1865,7 @@ void launch_correct_diag(tc_ctx *c) {
   hipLaunchKernelGGL(k_correct_diag, grid, dim3(64, 4), 0, c->stream, c->gd, c->f[TC_QT],
                      c->f[TC_QR], c->f[TC_QZ], c->f[TC_PR], c->f[TC_RHS], c->f[TC_DQ],
                      c->f[TC_QCAP], c->f[TC_DPH], c->dt, c->al, c->beta, c->mask, c->mask + c->P,
-                     c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red, correct_diag_sync(),
-                     correct_diag_xcd());
+                     c->p.virotint, c->p.virotext, del, deln, c->g.pi, c->red);
   std::swap(c->f[TC_QT], c->f[TC_RHS]);
   std::swap(c->f[TC_QR], c->f[TC_DQ]);
   std::swap(c->f[TC_QZ], c->f[TC_QCAP]);

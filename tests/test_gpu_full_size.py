@@ -16,6 +16,8 @@ and through properties that do not need the oracle:
 - the projection is idempotent: projecting a projected field changes it by
   roundoff only, and leaves the divergence at roundoff.
 """
+import importlib
+
 import numpy as np
 import pytest
 
@@ -154,3 +156,86 @@ def test_large_1024_fused_and_routine_paths_agree(tc):
         print(f"  fused={fused}: {out[-1]}", flush=True)
         dev.close()
     assert np.allclose(out[0], out[1], rtol=1e-11, atol=0), (out[0], out[1])
+
+
+def test_production_512_whole_run(tc):
+    """The production workload as specified, all 500 steps through the
+    library's main loop (tc_run_steps_logged): it ends with code 555 at
+    NTST, every step logs (TOUT below dt), every dt follows main.F90's rule
+    from the previous step's logged CFL (DTMIN for steps 0 and 1), the
+    divergence stays below RESID and the torques finite."""
+    p = _production(tc)
+    print("full size: production_512, 500 steps in the device main loop", flush=True)
+    qt, qr, qz = tc.init_perturbed_couette(p, tc.configs.SEEDS["production_512"],
+                                           tc.configs.AMPLITUDE["production_512"])
+    dev = tc.Solver(p)
+    dev.set("qt", qt); dev.set("qr", qr); dev.set("qz", qz)
+    del qt, qr, qz
+    dev.project()
+    code, last, d, rows = dev.run_logged(0, 1000)
+    print(f"  code {code} at ntime {last}, {len(rows)} logged steps, last {d}", flush=True)
+    assert (code, last) == (555, p["ntst"]) and len(rows) == p["ntst"] + 1
+    for (ntime, r), (_, prev) in zip(rows, [(None, None)] + rows[:-1]):
+        assert r["dmax"] < p["resid"], (ntime, r)
+        assert np.isfinite(r["torque_in"]) and np.isfinite(r["torque_out"]), (ntime, r)
+        if ntime <= 1:  # main.F90:157-163
+            assert r["dt"] == p["dtmin"], (ntime, r)
+        else:
+            assert r["dt"] == min(p["dtmax"], p["limitcfl"] / prev["cfl"]), (ntime, r, prev)
+    dev.close()
+
+
+@pytest.fixture(scope="module")
+def large_start(tc):
+    p = tc.configs.large_1024()
+    return tc.init_perturbed_couette(p, tc.configs.SEEDS["large_1024"],
+                                     tc.configs.AMPLITUDE["large_1024"])
+
+
+def _large_run(tc, p, start, nsteps):
+    dev = tc.Solver(p)
+    for name, a in zip(("qt", "qr", "qz"), start):
+        dev.set(name, a)
+    dev.project()
+    out = dev.run_logged(0, nsteps)
+    div = dev.CheckDivergence()
+    dev.close()
+    return out, div
+
+
+def test_large_1024_whole_run_at_the_stable_dt(tc, large_start):
+    """1024 x 1024 x 769 for all its 100 steps through the library's main
+    loop, with DTMAX at the largest dt the reference's incremental pressure
+    update keeps stable on this grid (sweep.pressure_stable_dt, about
+    5.2e-5, below the CFL = 1.2 dt of about 1.8e-4): code 555 at NTST, every
+    step logged with its divergence below RESID, and no velocity above the
+    driven wall's (1) by more than the perturbation."""
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    p = tc.configs.large_1024()
+    dt = sw.pressure_stable_dt(tc, p, p["dtmax"])
+    p.update(dtmax=dt, dtmin=1e-6, tout=1e-12)
+    print(f"1024x1024x769: 100 steps at dt <= {dt:.3e}", flush=True)
+    (code, last, d, rows), div = _large_run(tc, p, large_start, 1000)
+    print(f"  code {code} at ntime {last}, last {d}", flush=True)
+    assert (code, last) == (555, p["ntst"]) and len(rows) == p["ntst"] + 1
+    for ntime, r in rows:
+        assert r["dmax"] < p["resid"] and r["vmax1"] < 1.01, (ntime, r)
+        assert r["dt"] == (p["dtmin"] if ntime <= 1 else dt), (ntime, r)
+    assert div < p["resid"]
+
+
+def test_large_1024_as_configured_meets_the_dt_floor(tc, large_start):
+    """The same workload at its CFL = 1.2 dt: there the reference's pressure
+    update amplifies a radial pressure mode about 4.5 times per step (see
+    sweep.pressure_step_amplification), the velocity leaves the bounds of
+    the wall speeds within a few tens of steps, dt falls below DTMIN = 1e-4
+    and the main loop ends with code 166 (main.F90:165), before NTST.  The
+    512 x 512 x 385 analogue shows the same growth in the oracle (both grow
+    by the same factor per step)."""
+    p = tc.configs.large_1024()
+    p.update(tout=1e-12)  # log every step (diagnostics only)
+    print("1024x1024x769 as configured: runs into the dt floor", flush=True)
+    (code, last, d, rows), _ = _large_run(tc, p, large_start, 1000)
+    print(f"  code {code} at ntime {last}, last {d}", flush=True)
+    assert code == 166 and 5 <= last <= 40, (code, last, d)
+    assert rows[0][1]["vmax1"] < 1.01 and max(r["vmax1"] for _, r in rows) > 1.05, rows

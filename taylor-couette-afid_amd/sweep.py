@@ -28,7 +28,13 @@ streams, so small jobs do not wait for the step of a large one; each
 batch's step (all its grouped launches) is replayed as one hipGraph; jobs
 join and leave a batch at step boundaries, every ``chunk`` steps.  Host fields
 are prepared on worker threads while the device computes, and each job's
-time to result (arrival to final step) is recorded.
+time to result (arrival to final step) is recorded.  With a checkpoint
+directory, every job's whole state is written as it runs (the restart
+format of tc_write_restart) and a later run continues jobs from it.
+
+``batch_autotune`` picks a schedule per grid shape for a batch (timed on
+the whole group of that shape); ``run_one_at_a_time`` is the baseline that
+serves the same trace one job at a time on the single-instance path.
 """
 import concurrent.futures as cf
 import math

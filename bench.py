@@ -442,8 +442,9 @@ def summarize_profile(out_dir, workload, p):
              "5 timed steps + 1 warm-up step + 5 copies of one field.  Algorithmic GB/s =",
              "the kernel's distinct input and output arrays (bench.py kernel_bytes) / average",
              "duration.", "",
-             "| kernel | calls | average us | min us | max us | share % | algorithmic GB | GB/s |",
-             "|---|---|---|---|---|---|---|---|"]
+             "| kernel | calls | average us | min us | max us | share % | algorithmic GB | GB/s | "
+             "frac of 8 TB/s |",
+             "|---|---|---|---|---|---|---|---|---|"]
     for r in rows:
         k = _kernel_short(r["Name"])
         t = _timer_of(k)
@@ -453,8 +454,10 @@ def summarize_profile(out_dir, workload, p):
             b = 2 * copy_bytes
         gb = f"{b / 1e9:.3f}" if b else "-"
         gbs = f"{b / (avg * 1e-6) / 1e9:.0f}" if b else "-"
+        frac = f"{b / (avg * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f}" if b else "-"
         lines.append(f"| {k} | {r['Calls']} | {avg:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
-                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} | {gb} | {gbs} |")
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} | {gb} | {gbs} | "
+                     f"{frac} |")
     stats_path = os.path.join(PROFILES, f"all_kernels_{workload}_stats.md")
     open(stats_path, "w").write("\n".join(lines) + "\n")
 

@@ -279,3 +279,30 @@ def test_batch_checkpoint_continues_bit_for_bit(tc, tmp_path):
         for f in FIELDS:
             assert np.array_equal(s.get(f), w[f]), f
         s.close()
+
+
+def test_batch_autotune_returns_schedules_that_run(tc):
+    """sweep.batch_autotune on two small shape groups: a valid schedule per
+    shape, built on each shape's lone-instance tuning, and a batch with those
+    schedules steps like the single-instance path with the same schedule."""
+    import importlib
+    sw = importlib.import_module("taylor-couette-afid_amd.sweep")
+    mix = _mix(tc)
+    groups = {}
+    for p, _ in mix:
+        groups.setdefault((p["nthm"], p["nzm"], p["nrm"]), []).append(p)
+    tun = sw.batch_autotune(tc, groups, steps=2)
+    assert set(tun) == set(groups)
+    for sh, t in tun.items():
+        assert t["helm_lpt"] in (1, 2) and t["defer"] in (1, 2, -1), (sh, t)
+    p, seed = mix[0]
+    st = _start(tc, p, seed)
+    t = tun[(p["nthm"], p["nzm"], p["nrm"])]
+    with tc.Batch() as b:
+        i = b.add(p, t)
+        _load(b.solver(i), st)
+        b.advance(3, graph=True, sync_every=3)
+        got = {f: b.solver(i).get(f) for f in FIELDS}
+    ref, _ = _single(tc, p, st, 3, t)
+    for f in FIELDS:
+        assert np.array_equal(got[f], ref[f]), f

@@ -94,6 +94,29 @@ def test_variable_dt_steps_match_oracle(tc, oracle):
     _compare(dev, orc, p["nrm"])
 
 
+@pytest.mark.parametrize("tout", [1e30, 0.004])
+def test_variable_dt_without_logging_matches_oracle(tc, oracle, tout):
+    """Variable dt where most steps do not log (TOUT = 1e30: only ntime 1;
+    0.004: every few steps): the device takes each non-logging step's CFL
+    from the previous step's fused last correction pass instead of a CFL
+    pass of its own (CalcMaxCFL of main.F90:156 on the same fields); the dt
+    sequence and the fields match the oracle, which evaluates CalcMaxCFL at
+    every step start."""
+    p = tc.configs.scaled(tc.configs.production_512(), 48, 40, 33)
+    p.update(ntst=1000, tout=tout)
+    dev, orc = _start(tc, oracle, p, 17, 1e-2)
+    logged = 0
+    for n in range(12):
+        c1, d1 = dev.step(n)
+        c2, d2 = orc.step(n)
+        assert c1 == c2 == 0, (n, c1, c2)
+        assert abs(d1["dt"] / d2["dt"] - 1) < 1e-12, (n, d1["dt"], d2["dt"])
+        logged += d1["vmax1"] != 0.0
+    assert logged < 12
+    assert dev.CheckDivergence() < p["resid"]
+    _compare(dev, orc, p["nrm"])
+
+
 def test_small_64_full_run(tc, oracle):
     """The 64x64x65 workload as specified: RK3 at fixed dt 1e-3, NTST = 199,
     i.e. 200 steps ending with code 555 (main.F90:152,223), through the

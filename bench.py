@@ -36,19 +36,30 @@ _LEG = "--no-cpu --sweep-steps 0 --served-jobs 0"
 # the kernel the roofline object reports (the largest share of the step)
 DOMINANT = "helm_fwd_corr"
 
-PROFILE_COMMANDS = {
-    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o bench "
-             f"-- python3 bench.py --steps 5 --warmup 1 {_LEG}",
-    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE --output-format csv --kernel-include-regex "
-                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
-                  f"-d OUT/FETCH_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 {_LEG}",
-    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE --output-format csv --kernel-include-regex "
-                  "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
-                  f"-d OUT/WRITE_SIZE -o pmc -- python3 bench.py --steps 1 --warmup 1 {_LEG}",
-    # the batched sweep: 2 untimed + 6 timed graph-replayed steps of the 48-instance mix
-    "sweep": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/sweep -o sweep "
-             "-- python3 bench.py --steps 1 --warmup 1 --no-cpu --sweep-steps 6 --served-jobs 0",
-}
+def profile_commands(workload="production_512"):
+    """rocprofv3 commands whose output --summarize-profile turns into the
+    committed summaries (run from /tmp with TMPDIR=/tmp; OUT is the output
+    folder).  Other workloads than the default add --workload; the
+    1024x1024x769 one times 3 steps (it reaches the reference's dt floor
+    after 12, see configs.large_1024)."""
+    w = "" if workload == "production_512" else f" --workload {workload}"
+    nst = 3 if workload == "large_1024" else 5
+    pmc = ("rocprofv3 --pmc {c} --output-format csv --kernel-include-regex "
+           "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
+           f"-d OUT/{{c}} -o pmc -- python3 bench.py{w} --steps 1 --warmup 1 {_LEG}")
+    return {
+        "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o bench "
+                 f"-- python3 bench.py{w} --steps {nst} --warmup 1 {_LEG}",
+        "trace_steps": nst,
+        "FETCH_SIZE": pmc.format(c="FETCH_SIZE"),
+        "WRITE_SIZE": pmc.format(c="WRITE_SIZE"),
+        # the batched sweep: 2 untimed + 6 timed graph-replayed steps of the 48-instance mix
+        "sweep": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/sweep -o sweep "
+                 "-- python3 bench.py --steps 1 --warmup 1 --no-cpu --sweep-steps 6 --served-jobs 0",
+    }
+
+
+PROFILE_COMMANDS = profile_commands()
 
 
 def kernel_bytes(p):
@@ -431,6 +442,7 @@ def summarize_profile(out_dir, workload, p):
     import collections
 
     kb = kernel_bytes(p)
+    cmds = profile_commands(workload)
     N = p["nthm"] * p["nzm"] * p["nrm"]
     R = 8 * N
     copy_bytes = 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"]
@@ -438,8 +450,9 @@ def summarize_profile(out_dir, workload, p):
     rows = list(csv.DictReader(open(os.path.join(out_dir, "trace", "bench_kernel_stats.csv"))))
     lines = [f"# Kernel durations, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']}, theta x z x r)", "",
              "Command (from /tmp, TMPDIR=/tmp, OUT = an output folder, repository root as",
-             "the working directory of bench.py):", "", "    " + PROFILE_COMMANDS["trace"], "",
-             "5 timed steps + 1 warm-up step + 5 copies of one field.  Algorithmic GB/s =",
+             "the working directory of bench.py):", "", "    " + cmds["trace"], "",
+             f"{cmds['trace_steps']} timed steps + 1 warm-up step + {min(cmds['trace_steps'], 5)} "
+             "more steps with every kernel timed + 5 copies of one field.  Algorithmic GB/s =",
              "the kernel's distinct input and output arrays (bench.py kernel_bytes) / average",
              "duration.", "",
              "| kernel | calls | average us | min us | max us | share % | algorithmic GB | GB/s | "
@@ -474,7 +487,7 @@ def summarize_profile(out_dir, workload, p):
     kw = copy_bytes / (wr["k_copy"] * 1024)
     lines = [f"# HBM traffic per dispatch, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']})", "",
              "Commands (from /tmp, TMPDIR=/tmp, OUT = an output folder), one counter per run:", "",
-             "    " + PROFILE_COMMANDS["FETCH_SIZE"], "    " + PROFILE_COMMANDS["WRITE_SIZE"], "",
+             "    " + cmds["FETCH_SIZE"], "    " + cmds["WRITE_SIZE"], "",
              f"Calibration: the streaming copy of one field moves {copy_bytes} B each way; its",
              f"FETCH_SIZE reads {fe['k_copy']:.0f} KiB and its WRITE_SIZE {wr['k_copy']:.0f} KiB per "
              f"dispatch, so", f"read bytes = FETCH_SIZE x 1024 x {kf:.4f} and write bytes = WRITE_SIZE "

@@ -752,8 +752,11 @@ __global__ __launch_bounds__(512) void k_poisson_tridiag_g(const StepDesc *__res
 }
 
 // Launch shape of k_poisson_tridiag for a line of n rows: PT partitions of at
-// most PQ rows (and at least 2), LB lines per block with LB*PT <= 512
-// (<= 256 for PQ = 13, so that three blocks share a CU).
+// most PQ rows (and at least 2), LB lines per block with LB*PT <= 256
+// (TC_POISSON_MAXT, at most 512).  Smaller and larger blocks measured slower
+// for both row counts: at 512x512x385, 13 rows, 351 us against 452 (128) and
+// 594 (512); at 1024x1024x769, 26 rows, 3261 us against 4267 (128) and 3982
+// (512).
 // Dynamic LDS of k_poisson_tridiag: the padded reduced systems of lb lines
 // plus the gauge line (2n), its multipliers (n) and four coefficient tables.
 size_t poisson_lds_doubles(int pt, int lb, int n) {
@@ -768,7 +771,7 @@ void poisson_part_shape(int n, int PQ, int &pt, int &lb) {
     const char *e = std::getenv("TC_POISSON_MAXT");
     return e ? std::atoi(e) : 0;
   }();
-  const int maxt = env_maxt > 0 ? env_maxt : ((PQ <= 13) ? 256 : 512);
+  const int maxt = env_maxt > 0 ? std::min(env_maxt, 512) : 256;
   lb = 64;
   while (lb > 1 && lb * pt > maxt) lb /= 2;
 }

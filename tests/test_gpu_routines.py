@@ -113,6 +113,19 @@ def test_solve_pressure_correction_rows_per_thread(tc, oracle, shape, pq):
     _cmp(dev, orc, "dph", slice(0, nrm), tol=1e-11)
 
 
+# z-FFT block shapes: 256 and 512 (default) threads with 32 and 64 KiB of
+# LDS (column tiles of 1 to 64 ktheta), on radix-2/3/5 lengths
+@pytest.mark.parametrize("nt,lds", [(256, 32768), (512, 65536), (512, 32768), (256, 65536)])
+@pytest.mark.parametrize("shape", [(32, 24, 9), (40, 1024, 5), (30, 360, 7)])
+def test_solve_pressure_correction_z_blocks(tc, oracle, shape, nt, lds):
+    p, dev, orc = _setup(tc, oracle, shape, 47)
+    nrm = shape[2]
+    dev.set_tuning({"fft_z_nt": nt, "fft_z_lds": lds})
+    assert dev.tuning()["fft_z_nt"] == nt
+    dev.SolvePressureCorrection(); orc.call("solve_pressure_correction")
+    _cmp(dev, orc, "dph", slice(0, nrm), tol=1e-11)
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_reductions(tc, oracle, shape):
     p, dev, orc = _setup(tc, oracle, shape, 51, masks=True)

@@ -351,8 +351,9 @@ int tc_batch_groups(tc_batch *b, int *ngroups, int *launches_per_step);
 /* WriteFlowField.F90:11-53 / ReadFlowField.F90 for one batch instance: its
  * whole state (tc_write_restart's format) at its current step, and the
  * load of such a checkpoint into an instance that has not stepped yet, which
- * then continues from the checkpoint's step and time bit for bit (its fixed
- * dt must equal the checkpoint's). */
+ * then continues from the checkpoint's step and time bit for bit.  The load
+ * is refused, with the instance left as it was, when the checkpoint's dt is
+ * not the instance's fixed dt or its step is past the instance's NTST. */
 int tc_batch_write_restart(tc_batch *b, int id, const char *path);
 int tc_batch_read_restart(tc_batch *b, int id, const char *path, int *ntime);
 

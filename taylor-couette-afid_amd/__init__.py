@@ -12,6 +12,7 @@ if the library has not been built.
 """
 import ctypes
 import os
+import struct
 
 import numpy as np
 
@@ -364,7 +365,8 @@ class Solver:
         _check(lib().tc_project(self.h))
 
     def write_restart(self, path, ntime):
-        """Whole-state checkpoint (fields, pressure, RK histories, time)."""
+        """Whole-state checkpoint (fields, pressure, RK histories, time);
+        restart_info reads its header."""
         _check(lib().tc_write_restart(self.h, os.fsencode(path), int(ntime)))
 
     def read_restart(self, path):
@@ -484,6 +486,21 @@ class Solver:
         v = ctypes.c_ulonglong()
         _check(lib().tc_device_bytes(self.h, ctypes.byref(v)))
         return v.value
+
+
+_RESTART_HEADER = struct.Struct("<8s8i8d")
+
+
+def restart_info(path):
+    """The header of a checkpoint written by tc_write_restart (8-byte magic,
+    int32 nthm nzm nrm nsst ntime cfl_valid istr pad, float64 time dt cfl
+    rint rext alx3 lamb str), without reading its fields."""
+    with open(path, "rb") as f:
+        raw = f.read(_RESTART_HEADER.size)
+    if len(raw) != _RESTART_HEADER.size or raw[:8] != b"TCAFIDR1":
+        raise TCError(f"{path} is not a restart file of this library")
+    v = _RESTART_HEADER.unpack(raw)
+    return dict(nthm=v[1], nzm=v[2], nrm=v[3], nsst=v[4], ntime=v[5], time=v[9], dt=v[10])
 
 
 def autotune(params, steps=3):

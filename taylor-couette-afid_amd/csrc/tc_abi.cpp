@@ -743,8 +743,26 @@ struct RestartHeader {
   int32_t nthm, nzm, nrm, nsst, ntime, cfl_valid, istr, pad;
   double time, dt, cfl, rint, rext, alx3, lamb, str;
 };
+static_assert(sizeof(RestartHeader) == 104, "restart header layout");
 const int kRestartFields[] = {TC_QT, TC_QR, TC_QZ, TC_PR, TC_RU1, TC_RU2, TC_RU3};
 }  // namespace
+
+extern "C++" {
+namespace tc {
+// The dt and step a restart file was written with, from its header alone
+int restart_file_info(const char *path, double *dt, int *ntime) {
+  std::FILE *f = path ? std::fopen(path, "rb") : nullptr;
+  if (!f) return fail(std::string("cannot open ") + (path ? path : "(null)"));
+  RestartHeader h;
+  const bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, "TCAFIDR1", 8) == 0;
+  std::fclose(f);
+  if (!ok) return fail(std::string(path) + " is not a restart file of this library");
+  *dt = h.dt;
+  *ntime = h.ntime;
+  return 0;
+}
+}  // namespace tc
+}
 
 int tc_write_restart(tc_ctx *c, const char *path, int ntime) {
   CTXCHK(c);

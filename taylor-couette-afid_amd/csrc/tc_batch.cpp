@@ -548,11 +548,20 @@ int tc_batch_read_restart(tc_batch *b, int id, const char *path, int *ntime) {
   Inst &I = b->inst[(size_t)id];
   if (I.steps != 0 || I.done)
     return tc::set_error("tc_batch_read_restart: the instance has already stepped");
+  // checked before any field is replaced, so a refused file leaves the
+  // instance as it was
+  double fdt = 0.0;
+  int fnt = 0;
+  rc = tc::restart_file_info(path, &fdt, &fnt);
+  if (rc) return rc;
+  if (fdt != I.c->p.dtmax)
+    return tc::set_error("tc_batch_read_restart: the checkpoint's dt is not the instance's DTMAX");
+  if (fnt > I.c->p.ntst)
+    return tc::set_error("tc_batch_read_restart: the checkpoint is at step " + std::to_string(fnt) +
+                         ", past NTST = " + std::to_string(I.c->p.ntst) + ": nothing left to run");
   int nt = 0;
   rc = tc_read_restart(I.c, path, &nt);
   if (rc) return rc;
-  if (I.c->dt != I.c->p.dtmax)
-    return tc::set_error("tc_batch_read_restart: the checkpoint's dt is not the instance's DTMAX");
   I.steps = nt;
   I.time = I.c->time;
   if (ntime) *ntime = nt;

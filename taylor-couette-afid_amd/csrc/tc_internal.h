@@ -274,6 +274,8 @@ void launch_triad(tc_ctx *c, int a, int b, int cc, double s);
 // the substep, dyn/res: device arrays of n
 bool defers_correction(const tc_ctx *c);
 int set_deferral(tc_ctx *c, int mode);
+// the dt and step recorded in a restart file's header (nothing else is read)
+int restart_file_info(const char *path, double *dt, int *ntime);
 void launch_helmholtz_fused_g(tc_ctx *c, const StepDesc *tab, const BatchDyn *dyn, int ns, int n,
                               hipStream_t s, bool corr);
 void launch_correct_both_g(tc_ctx *c, const StepDesc *tab, int n, hipStream_t s);

@@ -37,6 +37,7 @@ the whole group of that shape); ``run_one_at_a_time`` is the baseline that
 serves the same trace one job at a time on the single-instance path.
 """
 import concurrent.futures as cf
+import json
 import math
 import os
 import threading
@@ -266,7 +267,7 @@ class _ClassWorker:
                     steps, code, _ = b.status(iid)
                     every = srv.checkpoint_every
                     if every and (code or steps // every > self.saved[iid] // every):
-                        srv._checkpoint(b, iid, self.active[iid])
+                        srv._checkpoint(b, iid, self.active[iid], end=(steps, code) if code else None)
                         self.saved[iid] = steps
                     if code:
                         q = self.active.pop(iid)
@@ -343,8 +344,11 @@ class Server:
         # checkpoint_dir / checkpoint_every: each job's whole state is written
         # to checkpoint_dir/job<index>.tcr (WriteFlowField's role) at the first
         # chunk boundary past every multiple of checkpoint_every steps and
-        # when it ends; run(resume=True) continues each job that has one
-        # from it, bit for bit the run that wrote it.
+        # when it ends, and at its end job<index>.end records its steps and
+        # exit code; run(resume=True) continues each job that has a
+        # checkpoint from it, bit for bit the run that wrote it, and reports
+        # a job that had already ended (no steps left, or ended by an error
+        # code) without running it again.
         if checkpoint_every and not checkpoint_dir:
             raise ValueError("checkpoint_every needs a checkpoint_dir")
         self.checkpoint_dir = checkpoint_dir
@@ -377,11 +381,40 @@ class Server:
     def checkpoint_path(self, q):
         return os.path.join(self.checkpoint_dir, f"job{q:04d}.tcr")
 
-    def _checkpoint(self, b, iid, q):
-        # written whole before it replaces the previous one
+    def _checkpoint(self, b, iid, q, end=None):
+        # written whole before it replaces the previous one; end = (steps,
+        # code) when the job has ended
         path = self.checkpoint_path(q)
         b.write_restart(iid, path + ".tmp")
         os.replace(path + ".tmp", path)
+        if end is not None:
+            ep = self.end_path(q)
+            with open(ep + ".tmp", "w") as f:
+                json.dump({"steps": int(end[0]), "code": int(end[1])}, f)
+            os.replace(ep + ".tmp", ep)
+
+    def end_path(self, q):
+        return os.path.join(self.checkpoint_dir, f"job{q:04d}.end")
+
+    def _ended_before(self, q, p):
+        """(steps, code) of a job whose checkpoints say it has nothing left
+        to run, else None: its last checkpoint is past NTST, or it is the
+        one written when the job ended by an exit code other than 555."""
+        ck = self.checkpoint_path(q)
+        if not os.path.exists(ck):
+            return None
+        n = self.tc.restart_info(ck)["ntime"]
+        end = None
+        if os.path.exists(self.end_path(q)):
+            with open(self.end_path(q)) as f:
+                end = json.load(f)
+            if end["steps"] != n:
+                end = None  # from an earlier end, before the job was extended
+        if n > p["ntst"]:
+            return (n, end["code"] if end else 555)
+        if end and end["code"] != 555:
+            return (n, end["code"])
+        return None
 
     def _finished(self, q, steps, code, fields):
         with self.cv:
@@ -414,6 +447,18 @@ class Server:
         self.t0 = time.perf_counter()
         workers = {}
         waiting = sorted(range(len(jobs)), key=lambda q: jobs[q][0])
+        if resume:
+            for q in list(waiting):
+                ended = self._ended_before(q, jobs[q][1])
+                if ended:
+                    p0 = jobs[q][1]
+                    self.rec[q] = dict(arrival=jobs[q][0], admitted=None, done=0.0,
+                                       steps=ended[0], code=ended[1], resumed_at=ended[0],
+                                       cells=p0["nthm"] * p0["nzm"] * p0["nrm"],
+                                       steps_total=p0["ntst"] + 1)
+                    if keep_fields:
+                        self.rec[q]["fields"] = None
+                    waiting.remove(q)
         futures = {}
         try:
             while True:

@@ -94,6 +94,24 @@ def _has_gpu(tc):
         return False
 
 
+def test_restart_info_reads_the_documented_header(tc, tmp_path):
+    """restart_info parses the header layout include/tc_abi.h documents for
+    tc_write_restart files and rejects other files."""
+    import struct
+    path = tmp_path / "x.tcr"
+    hdr = struct.pack("<8s8i8d", b"TCAFIDR1", 64, 48, 33, 3, 120, 1, 4, 0,
+                      0.75, 2.5e-3, 0.4, 1.0, 2.0, 6.2831853, 1.0, 1.2)
+    path.write_bytes(hdr + np.zeros(8).tobytes())
+    assert tc.restart_info(str(path)) == dict(nthm=64, nzm=48, nrm=33, nsst=3, ntime=120,
+                                              time=0.75, dt=2.5e-3)
+    path.write_bytes(b"TCAFIDR0" + hdr[8:])
+    with pytest.raises(tc.TCError):
+        tc.restart_info(str(path))
+    path.write_bytes(hdr[:50])
+    with pytest.raises(tc.TCError):
+        tc.restart_info(str(path))
+
+
 def test_no_silent_fallback_without_device(tc):
     if _has_gpu(tc):
         pytest.skip("a GPU is visible")

@@ -266,6 +266,24 @@ def test_batch_checkpoint_continues_bit_for_bit(tc, tmp_path):
         with pytest.raises(tc.TCError):
             b.read_restart(extra, paths[1])  # another grid
         b.remove(extra)
+        # another fixed dt: refused from the header, before any field moves
+        p2 = dict(mix[0][0])
+        p2.update(dtmax=1e-3, dtmin=1e-3)
+        extra = b.add(p2)
+        _load(b.solver(extra), starts[0])
+        with pytest.raises(tc.TCError, match="DTMAX"):
+            b.read_restart(extra, paths[0])
+        for f, v in starts[0].items():
+            assert np.array_equal(b.solver(extra).get(f), v), f
+        b.remove(extra)
+        # a checkpoint past the instance's NTST: nothing left to run
+        extra = b.add(dict(mix[0][0], ntst=5))
+        with pytest.raises(tc.TCError, match="past NTST"):
+            b.read_restart(extra, paths[0])
+        b.remove(extra)
+        info = tc.restart_info(paths[0])
+        assert (info["nthm"], info["nzm"], info["nrm"], info["ntime"]) == (48, 32, 25, 6)
+        assert info["dt"] == 2e-3 and abs(info["time"] - 6 * 2e-3) < 1e-15
         b.advance(5, graph=True, sync_every=2)
         for i, w in zip(ids, want):
             assert b.status(i)[:2] == (11, 0)

@@ -222,8 +222,10 @@ def test_batch_autotune_times_each_candidate_on_the_whole_group(tc):
 
 def test_checkpoints_follow_the_cadence_and_resume_continues(tc, tmp_path):
     """checkpoint_every: a job's state is written at the first chunk boundary
-    past every multiple and when it ends (each write replacing the last);
-    run(resume=True) continues a job from its checkpoint's step."""
+    past every multiple and when it ends (each write replacing the last, the
+    end also recorded with its exit code); run(resume=True) continues a job
+    from its checkpoint's step, and reports a job with nothing left to run,
+    or one that ended by an error code, without running it."""
     import json
     import os
     sw = importlib.import_module("taylor-couette-afid_amd.sweep")
@@ -241,18 +243,36 @@ def test_checkpoints_follow_the_cadence_and_resume_continues(tc, tmp_path):
             self.inst[i]["steps"] = n
             return n
 
+    def restart_info(path):  # the fake checkpoints' header
+        with open(path) as f:
+            return {"ntime": json.load(f)["steps"]}
+
     ftc = FakeTC(tc)
     ftc.Batch = CkptBatch
+    ftc.restart_info = restart_info
     g = (16, 16, 9)
     jobs = _jobs(sw, [g], [0.0], [20])  # ntst 19: 20 steps
     srv = sw.Server(ftc, chunk=4, workers=1, checkpoint_dir=str(tmp_path), checkpoint_every=6)
     rec = srv.run(jobs)
     assert rec[0]["code"] == 555 and rec[0]["steps"] == 20
     assert writes == [8, 12, 20], writes  # past 6, past 12, the end
-    assert sorted(os.listdir(tmp_path)) == ["job0000.tcr"]
+    assert sorted(os.listdir(tmp_path)) == ["job0000.end", "job0000.tcr"]
+    with open(tmp_path / "job0000.end") as f:
+        assert json.load(f) == {"steps": 20, "code": 555}
+    # resumed as it was: nothing left to run, reported without stepping
+    rec = srv.run(jobs, resume=True)
+    assert (rec[0]["steps"], rec[0]["code"], rec[0]["resumed_at"]) == (20, 555, 20)
+    assert writes == [8, 12, 20], writes
     # the same job extended to 40 steps continues from step 20
     longer = [(t, dict(p, ntst=39), s) for t, p, s in jobs]
     rec = srv.run(longer, resume=True)
     assert rec[0]["resumed_at"] == 20 and rec[0]["steps"] == 40 and rec[0]["code"] == 555
+    assert writes[-1] == 40
+    # a job that ended by an error code is not run again, even extended
+    with open(tmp_path / "job0000.end", "w") as f:
+        json.dump({"steps": 40, "code": 165}, f)
+    rec = srv.run([(t, dict(p, ntst=59), s) for t, p, s in jobs], resume=True)
+    assert (rec[0]["steps"], rec[0]["code"], rec[0]["resumed_at"]) == (40, 165, 40)
+    assert writes[-1] == 40
     with pytest.raises(ValueError):
         sw.Server(ftc, checkpoint_every=5)

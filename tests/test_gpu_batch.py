@@ -102,6 +102,31 @@ def test_batch_matches_single_instance_bitwise(tc, monkeypatch, defer):
                 assert abs(d[k] / d1[k] - 1) < 1e-13, (k, d[k], d1[k])
 
 
+def test_odd_shapes_in_a_batch_match_single_instance(tc):
+    """Odd theta or z sizes in one batch (the grouped kernels' paths for odd
+    nthm and theta-FFT row pairs that straddle r-planes), two instances of
+    one odd shape in a group: bitwise the single-instance runs."""
+    mix = [(_params(tc, 27, 20, 13, ren=700.0), 31), (_params(tc, 27, 20, 13, ren=900.0), 32),
+           (_params(tc, 40, 45, 17, ren=800.0, rext=1.5), 33),
+           (_params(tc, 45, 27, 9, ren=600.0), 34)]
+    starts = [_start(tc, p, seed) for p, seed in mix]
+    nsteps = 4
+    with tc.Batch() as b:
+        ids = []
+        for (p, _), st in zip(mix, starts):
+            i = b.add(p)
+            _load(b.solver(i), st)
+            ids.append(i)
+        assert b.groups()[0] == 3
+        assert b.advance(nsteps, graph=True, sync_every=2) == len(mix)
+        for (p, _), st, i in zip(mix, starts, ids):
+            ref, _ = _single(tc, p, st, nsteps)
+            assert b.status(i)[:2] == (nsteps, 0)
+            for f in FIELDS:
+                a = b.solver(i).get(f)
+                assert np.array_equal(a, ref[f]), (i, f, np.abs(a - ref[f]).max())
+
+
 def test_batch_graph_replays_match_eager(tc):
     mix = _mix(tc)[:3]
     starts = [_start(tc, p, seed) for p, seed in mix]

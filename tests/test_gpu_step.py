@@ -51,6 +51,29 @@ def test_steps_match_oracle_small(tc, oracle, fused):
     _compare(dev, orc, p["nrm"])
 
 
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("shape", [(40, 45, 17), (27, 20, 13), (45, 27, 9)])
+def test_odd_shapes_match_oracle(tc, oracle, shape, fused):
+    """Odd theta or z sizes (3- and 5-smooth): theta-FFT row pairs that
+    straddle two r-planes and a last unpaired row (odd nzm*nrm), and, for odd
+    nthm, the paths without theta-paired corrections or the fused
+    end-of-step pass."""
+    p = tc.configs.scaled(tc.configs.small_64(), *shape)
+    p.update(sizepert=0.1, omegapert=0.5, virotext=-0.3)
+    dev, orc = _start(tc, oracle, p, 23, 1e-2)
+    dev.set_fused(fused)
+    for n in range(5):
+        c1, d1 = dev.step(n)
+        c2, d2 = orc.step(n)
+        assert c1 == c2
+        assert d1["dt"] == d2["dt"]
+        if d2["dmax"] > 0:
+            assert d1["dmax"] < p["resid"]
+            assert abs(d1["torque_in"] / d2["torque_in"] - 1) < 1e-9
+            assert abs(d1["torque_out"] / d2["torque_out"] - 1) < 1e-9
+    _compare(dev, orc, p["nrm"])
+
+
 @pytest.mark.parametrize("defer", ["0", "2"])
 @pytest.mark.parametrize("lpt", ["1", "2"])
 @pytest.mark.parametrize("masks", [False, True])

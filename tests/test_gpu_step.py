@@ -52,12 +52,13 @@ def test_steps_match_oracle_small(tc, oracle, fused):
 
 
 @pytest.mark.parametrize("fused", [True, False])
-@pytest.mark.parametrize("shape", [(40, 45, 17), (27, 20, 13), (45, 27, 9)])
+@pytest.mark.parametrize("shape", [(40, 45, 17), (27, 20, 13), (45, 27, 9), (16, 12, 3), (8, 4, 5)])
 def test_odd_shapes_match_oracle(tc, oracle, shape, fused):
     """Odd theta or z sizes (3- and 5-smooth): theta-FFT row pairs that
     straddle two r-planes and a last unpaired row (odd nzm*nrm), and, for odd
     nthm, the paths without theta-paired corrections or the fused
-    end-of-step pass."""
+    end-of-step pass; the smallest radial size (nrm = 3: one Poisson
+    partition per line) and a theta size below one tile width."""
     p = tc.configs.scaled(tc.configs.small_64(), *shape)
     p.update(sizepert=0.1, omegapert=0.5, virotext=-0.3)
     dev, orc = _start(tc, oracle, p, 23, 1e-2)

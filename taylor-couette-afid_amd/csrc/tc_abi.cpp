@@ -21,16 +21,23 @@ namespace {
 
 thread_local std::string g_err;
 
+// Every reported failure also clears the runtime's last-error state: a HIP
+// call that failed on the way (an allocation, a copy) must not surface again
+// at the next launch check as a failed kernel launch.
 int fail(const std::string &msg) {
   g_err = msg;
+  (void)hipGetLastError();
   return -1;
 }
 
+// A failed call is reported here and cleared from the runtime's last-error
+// state, as fail() does.
 #define HIPCHK(call)                                                                     \
   do {                                                                                   \
     hipError_t e_ = (call);                                                              \
     if (e_ != hipSuccess) {                                                              \
       g_err = std::string(#call) + ": " + hipGetErrorString(e_);                         \
+      (void)hipGetLastError();                                                           \
       return -2;                                                                         \
     }                                                                                    \
   } while (0)
@@ -482,6 +489,7 @@ int tc_create_tuned(const tc_params *p, const tc_tuning *tun, tc_ctx **out) {
     hipError_t e_ = (x);                                                 \
     if (e_ != hipSuccess) {                                              \
       g_err = std::string(#x) + ": " + hipGetErrorString(e_);            \
+      (void)hipGetLastError();                                           \
       destroy_ctx(c);                                                    \
       return -2;                                                         \
     }                                                                    \

@@ -66,11 +66,14 @@ Key group_key(const tc_ctx *c) {
              tc::defers_correction(c) ? 1 : 0);
 }
 
+// a failed call is also cleared from the runtime's last-error state
 #define BCHK(x)                                                                  \
   do {                                                                           \
     hipError_t e_ = (x);                                                         \
-    if (e_ != hipSuccess)                                                        \
+    if (e_ != hipSuccess) {                                                      \
+      (void)hipGetLastError();                                                   \
       return tc::set_error(std::string(#x) + ": " + hipGetErrorString(e_));      \
+    }                                                                            \
   } while (0)
 
 }  // namespace
@@ -337,6 +340,7 @@ int tc_batch_create(tc_batch **out) {
   for (int k = 0; k < tc_batch::RING && e == hipSuccess; k++)
     e = hipEventCreateWithFlags(&b->dyn_ev[k], hipEventDisableTiming);
   if (e != hipSuccess) {
+    (void)hipGetLastError();
     tc_batch_destroy(b);
     return tc::set_error(std::string("tc_batch_create: ") + hipGetErrorString(e));
   }

@@ -1542,8 +1542,10 @@ static int scratch_on(int dev, double **out) {
   void *ptr = nullptr;
   const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
   if (cur != dev) (void)hipSetDevice(cur);
-  if (e != hipSuccess || !ptr)
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
     return set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
   scratch_dev[dev] = static_cast<double *>(ptr);
   *out = static_cast<double *>(ptr);
   return 0;

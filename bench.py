@@ -665,7 +665,19 @@ def main():
         "step_roofline": {"bound": "hbm", "bytes_per_step": round(step_gb * 1e9),
                           "field_passes": round(step_gb / (8 * npts / 1e9), 1),
                           "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                          # every kernel of the step: average launch (HIP events
+                          # on the context's stream: the dominant kernel over the
+                          # timed steps, the others over up to 5 more steps),
+                          # algorithmic bytes, fraction of the HBM peak
+                          "kernels": {
+                              n: {"avg_launch_ms": round(ms / cnt, 4),
+                                  "launches_per_step": round(cnt / a.steps, 3),
+                                  "bytes_per_launch": kb[n],
+                                  "achieved_gbs": round(kb[n] / (ms / cnt / 1e3) / 1e9, 1),
+                                  "frac": round(kb[n] / (ms / cnt / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+                              for n, (ms, cnt) in sorted(ktimes.items(), key=lambda t: -t[1][0])
+                              if n in kb and cnt > 0}},
     }
     if a.sweep_steps > 0:
         out["sweep"] = sweep_leg(tc, a.sweep_steps, value)

@@ -36,6 +36,17 @@ _LEG = "--no-cpu --sweep-steps 0 --served-jobs 0"
 # the kernel the roofline object reports (the largest share of the step)
 DOMINANT = "helm_fwd_corr"
 
+# counters of the two shader-sequencer passes (per-wave instruction mix;
+# where the waves' cycles go)
+SQ_PASSES = {
+    "SQ_MIX": ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD",
+               "SQ_INSTS_VMEM_WR", "SQ_INSTS_SMEM", "SQ_WAVE_CYCLES"],
+    "SQ_STALL": ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY",
+                 "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_VMEM", "SQ_LDS_BANK_CONFLICT"],
+    # (SQ_ACTIVE_INST_VMEM reads zero for every kernel here and is not summarised)
+}
+
+
 def profile_commands(workload="production_512"):
     """rocprofv3 commands whose output --summarize-profile turns into the
     committed summaries (run from /tmp with TMPDIR=/tmp; OUT is the output
@@ -47,12 +58,18 @@ def profile_commands(workload="production_512"):
     pmc = ("rocprofv3 --pmc {c} --output-format csv --kernel-include-regex "
            "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div|k_diagnostics\" "
            f"-d OUT/{{c}} -o pmc -- python3 bench.py{w} --steps 1 --warmup 1 {_LEG}")
+    sq = ("rocprofv3 --pmc {c} --output-format csv --kernel-include-regex "
+          "\"k_helm|k_copy|k_correct|k_fft|k_poisson|k_div\" "
+          f"-d OUT/{{n}} -o pmc -- python3 bench.py{w} --steps 1 --warmup 1 {_LEG}")
     return {
         "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o bench "
                  f"-- python3 bench.py{w} --steps {nst} --warmup 1 {_LEG}",
         "trace_steps": nst,
         "FETCH_SIZE": pmc.format(c="FETCH_SIZE"),
         "WRITE_SIZE": pmc.format(c="WRITE_SIZE"),
+        # shader-sequencer counters, two passes of at most eight
+        "SQ_MIX": sq.format(n="SQ_MIX", c=" ".join(SQ_PASSES["SQ_MIX"])),
+        "SQ_STALL": sq.format(n="SQ_STALL", c=" ".join(SQ_PASSES["SQ_STALL"])),
         # the batched sweep: 2 untimed + 6 timed graph-replayed steps of the 48-instance mix
         "sweep": "rocprofv3 --kernel-trace --stats --output-format csv -d OUT/sweep -o sweep "
                  "-- python3 bench.py --steps 1 --warmup 1 --no-cpu --sweep-steps 6 --served-jobs 0",
@@ -447,68 +464,125 @@ def summarize_profile(out_dir, workload, p):
     R = 8 * N
     copy_bytes = 8 * (p["nrm"] + 1) * p["nzm"] * p["nthm"]
 
-    rows = list(csv.DictReader(open(os.path.join(out_dir, "trace", "bench_kernel_stats.csv"))))
-    lines = [f"# Kernel durations, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']}, theta x z x r)", "",
-             "Command (from /tmp, TMPDIR=/tmp, OUT = an output folder, repository root as",
-             "the working directory of bench.py):", "", "    " + cmds["trace"], "",
-             f"{cmds['trace_steps']} timed steps + 1 warm-up step + {min(cmds['trace_steps'], 5)} "
-             "more steps with every kernel timed + 5 copies of one field.  Algorithmic GB/s =",
-             "the kernel's distinct input and output arrays (bench.py kernel_bytes) / average",
-             "duration.", "",
-             "| kernel | calls | average us | min us | max us | share % | algorithmic GB | GB/s | "
-             "frac of 8 TB/s |",
-             "|---|---|---|---|---|---|---|---|---|"]
-    for r in rows:
-        k = _kernel_short(r["Name"])
-        t = _timer_of(k)
-        avg = float(r["AverageNs"]) / 1e3
-        b = kb.get(t) or kb.get(t + "_fwd")
-        if t == "copy":
-            b = 2 * copy_bytes
-        gb = f"{b / 1e9:.3f}" if b else "-"
-        gbs = f"{b / (avg * 1e-6) / 1e9:.0f}" if b else "-"
-        frac = f"{b / (avg * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f}" if b else "-"
-        lines.append(f"| {k} | {r['Calls']} | {avg:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
-                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} | {gb} | {gbs} | "
-                     f"{frac} |")
-    stats_path = os.path.join(PROFILES, f"all_kernels_{workload}_stats.md")
-    open(stats_path, "w").write("\n".join(lines) + "\n")
+    trace_csv = os.path.join(out_dir, "trace", "bench_kernel_stats.csv")
+    if os.path.exists(trace_csv):
+        rows = list(csv.DictReader(open(trace_csv)))
+        lines = [f"# Kernel durations, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']}, theta x z x r)", "",
+                 "Command (from /tmp, TMPDIR=/tmp, OUT = an output folder, repository root as",
+                 "the working directory of bench.py):", "", "    " + cmds["trace"], "",
+                 f"{cmds['trace_steps']} timed steps + 1 warm-up step + {min(cmds['trace_steps'], 5)} "
+                 "more steps with every kernel timed + 5 copies of one field.  Algorithmic GB/s =",
+                 "the kernel's distinct input and output arrays (bench.py kernel_bytes) / average",
+                 "duration.", "",
+                 "| kernel | calls | average us | min us | max us | share % | algorithmic GB | GB/s | "
+                 "frac of 8 TB/s |",
+                 "|---|---|---|---|---|---|---|---|---|"]
+        for r in rows:
+            k = _kernel_short(r["Name"])
+            t = _timer_of(k)
+            avg = float(r["AverageNs"]) / 1e3
+            b = kb.get(t) or kb.get(t + "_fwd")
+            if t == "copy":
+                b = 2 * copy_bytes
+            gb = f"{b / 1e9:.3f}" if b else "-"
+            gbs = f"{b / (avg * 1e-6) / 1e9:.0f}" if b else "-"
+            frac = f"{b / (avg * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f}" if b else "-"
+            lines.append(f"| {k} | {r['Calls']} | {avg:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                         f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} | {gb} | {gbs} | "
+                         f"{frac} |")
+        stats_path = os.path.join(PROFILES, f"all_kernels_{workload}_stats.md")
+        open(stats_path, "w").write("\n".join(lines) + "\n")
 
-    def load(counter):
-        d = collections.defaultdict(list)
-        path = os.path.join(out_dir, counter, "pmc_counter_collection.csv")
-        for r in csv.DictReader(open(path)):
-            d[_kernel_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in d.items()}, {k: len(v) for k, v in d.items()}
+        print(stats_path)
+    if os.path.exists(os.path.join(out_dir, "FETCH_SIZE", "pmc_counter_collection.csv")):
+        def load(counter):
+            d = collections.defaultdict(list)
+            path = os.path.join(out_dir, counter, "pmc_counter_collection.csv")
+            for r in csv.DictReader(open(path)):
+                d[_kernel_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+            return {k: sum(v) / len(v) for k, v in d.items()}, {k: len(v) for k, v in d.items()}
 
-    fe, cnt = load("FETCH_SIZE")
-    wr, _ = load("WRITE_SIZE")
-    kf = copy_bytes / (fe["k_copy"] * 1024)
-    kw = copy_bytes / (wr["k_copy"] * 1024)
-    lines = [f"# HBM traffic per dispatch, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']})", "",
-             "Commands (from /tmp, TMPDIR=/tmp, OUT = an output folder), one counter per run:", "",
-             "    " + cmds["FETCH_SIZE"], "    " + cmds["WRITE_SIZE"], "",
-             f"Calibration: the streaming copy of one field moves {copy_bytes} B each way; its",
-             f"FETCH_SIZE reads {fe['k_copy']:.0f} KiB and its WRITE_SIZE {wr['k_copy']:.0f} KiB per "
-             f"dispatch, so", f"read bytes = FETCH_SIZE x 1024 x {kf:.4f} and write bytes = WRITE_SIZE "
-             f"x 1024 x {kw:.4f}.", f"One field pass (8 B x {N} cells) = {R / 1e9:.4f} GB.", "",
-             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per dispatch | "
-             "field passes | algorithmic passes |", "|---|---|---|---|---|---|---|"]
-    for k in sorted(fe, key=lambda k: -fe[k] - wr.get(k, 0)):
-        t = _timer_of(k)
-        b = kb.get(t) or kb.get(t + "_fwd")
-        if t == "copy":
-            b = 2 * copy_bytes
-        hbm = fe[k] * 1024 * kf + wr.get(k, 0.0) * 1024 * kw
-        lines.append(f"| {k} | {cnt[k]} | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {hbm:.0f} | "
-                     f"{hbm / R:.2f} | {(b / R) if b else float('nan'):.2f} |")
-    traffic_path = os.path.join(PROFILES, f"all_kernels_{workload}_hbm_traffic.md")
-    open(traffic_path, "w").write("\n".join(lines) + "\n")
-    print(stats_path)
-    print(traffic_path)
+        fe, cnt = load("FETCH_SIZE")
+        wr, _ = load("WRITE_SIZE")
+        kf = copy_bytes / (fe["k_copy"] * 1024)
+        kw = copy_bytes / (wr["k_copy"] * 1024)
+        lines = [f"# HBM traffic per dispatch, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']})", "",
+                 "Commands (from /tmp, TMPDIR=/tmp, OUT = an output folder), one counter per run:", "",
+                 "    " + cmds["FETCH_SIZE"], "    " + cmds["WRITE_SIZE"], "",
+                 f"Calibration: the streaming copy of one field moves {copy_bytes} B each way; its",
+                 f"FETCH_SIZE reads {fe['k_copy']:.0f} KiB and its WRITE_SIZE {wr['k_copy']:.0f} KiB per "
+                 f"dispatch, so", f"read bytes = FETCH_SIZE x 1024 x {kf:.4f} and write bytes = WRITE_SIZE "
+                 f"x 1024 x {kw:.4f}.", f"One field pass (8 B x {N} cells) = {R / 1e9:.4f} GB.", "",
+                 "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per dispatch | "
+                 "field passes | algorithmic passes |", "|---|---|---|---|---|---|---|"]
+        for k in sorted(fe, key=lambda k: -fe[k] - wr.get(k, 0)):
+            t = _timer_of(k)
+            b = kb.get(t) or kb.get(t + "_fwd")
+            if t == "copy":
+                b = 2 * copy_bytes
+            hbm = fe[k] * 1024 * kf + wr.get(k, 0.0) * 1024 * kw
+            lines.append(f"| {k} | {cnt[k]} | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {hbm:.0f} | "
+                         f"{hbm / R:.2f} | {(b / R) if b else float('nan'):.2f} |")
+        traffic_path = os.path.join(PROFILES, f"all_kernels_{workload}_hbm_traffic.md")
+        open(traffic_path, "w").write("\n".join(lines) + "\n")
+        print(traffic_path)
+    if os.path.exists(os.path.join(out_dir, "SQ_MIX", "pmc_counter_collection.csv")):
+        print(summarize_sq(out_dir, workload, p))
     sweep_csv = os.path.join(out_dir, "sweep", "sweep_kernel_trace.csv")
     if os.path.exists(sweep_csv):
         print(summarize_sweep(sweep_csv))
+
+
+def summarize_sq(out_dir, workload, p):
+    """Per-kernel shader-sequencer counters from the two SQ passes: the
+    instructions one wave issues, and the share of the waves' cycles spent
+    waiting, issuing VALU and vector-memory work.  Cycle counters are in the
+    same units (quad cycles), so their ratios are fractions of wave time."""
+    import csv
+    import collections
+
+    cmds = profile_commands(workload)
+
+    def load(name):
+        d = collections.defaultdict(lambda: collections.defaultdict(list))
+        path = os.path.join(out_dir, name, "pmc_counter_collection.csv")
+        for r in csv.DictReader(open(path)):
+            d[_kernel_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in d.items()}
+
+    mix, stall = load("SQ_MIX"), load("SQ_STALL")
+    lines = [f"# Shader-sequencer counters, {workload} ({p['nthm']}x{p['nzm']}x{p['nrm']})", "",
+             "Commands (from /tmp, TMPDIR=/tmp, OUT = an output folder), one pass each:", "",
+             "    " + cmds["SQ_MIX"], "    " + cmds["SQ_STALL"], "",
+             "Per wave: instructions issued by one wave (counter / SQ_WAVES, averaged over",
+             "dispatches).  Wait and VALU: SQ_WAIT_ANY and SQ_ACTIVE_INST_VALU as fractions",
+             "of SQ_WAVE_CYCLES (per wave; with w waves per SIMD the SIMD's VALU is busy",
+             "about w times that).  LDS conflict: cycles of SQ_LDS_BANK_CONFLICT per LDS",
+             "instruction.  The routine-level kernels of the set-up projection",
+             "(k_correct_velocity, k_divergence) appear as well.", "",
+             "| kernel | waves | VALU / wave | SALU / wave | LDS / wave | VMEM rd / wave | "
+             "VMEM wr / wave | SMEM / wave | wait | VALU active | "
+             "LDS conflict cycles / LDS instr |",
+             "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for k in sorted(mix, key=lambda k: -mix[k].get("SQ_WAVE_CYCLES", 0.0)):
+        m, st = mix[k], stall.get(k, {})
+        w = m.get("SQ_WAVES", 0.0)
+        if w <= 0:
+            continue
+        per = {c: m.get(c, 0.0) / w for c in SQ_PASSES["SQ_MIX"]}
+        wc = st.get("SQ_WAVE_CYCLES", 0.0)
+        fr = {c: (st.get(c, 0.0) / wc if wc > 0 else float("nan"))
+              for c in ("SQ_WAIT_ANY", "SQ_ACTIVE_INST_VALU")}
+        lds_w = m.get("SQ_INSTS_LDS", 0.0) / w
+        ws = st.get("SQ_WAVES", 0.0)
+        conf = (st.get("SQ_LDS_BANK_CONFLICT", 0.0) / ws / lds_w) if ws > 0 and lds_w > 0 else 0.0
+        lines.append(f"| {k} | {w:.0f} | {per['SQ_INSTS_VALU']:.0f} | {per['SQ_INSTS_SALU']:.0f} | "
+                     f"{lds_w:.0f} | {per['SQ_INSTS_VMEM_RD']:.0f} | {per['SQ_INSTS_VMEM_WR']:.0f} | "
+                     f"{per['SQ_INSTS_SMEM']:.0f} | {fr['SQ_WAIT_ANY']:.2f} | "
+                     f"{fr['SQ_ACTIVE_INST_VALU']:.2f} | {conf:.2f} |")
+    path = os.path.join(PROFILES, f"sq_counters_{workload}.md")
+    open(path, "w").write("\n".join(lines) + "\n")
+    return path
 
 
 def summarize_sweep(path):
